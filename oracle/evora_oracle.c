@@ -1,0 +1,558 @@
+/*
+ * evora_oracle.c -- CPU restatement of evora's hydro step (TEST INFRASTRUCTURE).
+ *
+ * The checker for the HIP library, never the product.  Every function keeps
+ * the reference's operand order so that, compiled without FP contraction,
+ * it reproduces the compiled Fortran bit for bit (pinned against oracle/_ref
+ * by tests/test_oracle_ref.py).  OpenMP only splits independent cells or
+ * pencils; per-cell arithmetic does not depend on the thread count.
+ */
+#include "evora_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NQ 6
+#define NW 5
+
+int ev_err_cell[3];
+
+/* ---- index helpers (Fortran bounds from parallel.f90:33-64, one rank) ---- */
+/* w(nw, -1:nx+2, -1:ny+2, -1:nz+2) */
+static inline long widx(const ev_params *p, int i, int j, int k) {
+  return (long)NW * ((long)(i + 2 - 1) + (long)(p->nx + 4) * ((long)(j + 1) + (long)(p->ny + 4) * (long)(k + 1)));
+}
+/* evo(nw, 0:nx+1, ...) and s(nw, 3, 0:nx+1, ...) share the cell index */
+static inline long eidx(const ev_params *p, int i, int j, int k) {
+  return (long)i + (long)(p->nx + 2) * ((long)j + (long)(p->ny + 2) * (long)k);
+}
+/* q(nq, 1:nx, 1:ny, 1:nz) */
+static inline long qidx(const ev_params *p, int i, int j, int k) {
+  return (long)NQ * ((long)(i - 1) + (long)p->nx * ((long)(j - 1) + (long)p->ny * (long)(k - 1)));
+}
+
+long ev_size_q(const ev_params *p) { return (long)NQ * p->nx * p->ny * p->nz; }
+long ev_size_w(const ev_params *p) { return (long)NW * (p->nx + 4) * (p->ny + 4) * (p->nz + 4); }
+long ev_size_evo(const ev_params *p) { return (long)NW * (p->nx + 2) * (p->ny + 2) * (p->nz + 2); }
+long ev_size_s(const ev_params *p) { return 3L * NW * (p->nx + 2) * (p->ny + 2) * (p->nz + 2); }
+
+static void set_err_cell(int i, int j, int k) {
+  ev_err_cell[0] = i;
+  ev_err_cell[1] = j;
+  ev_err_cell[2] = k;
+}
+
+static inline double fmin2(double a, double b) { return a < b ? a : b; }
+static inline double fmax2(double a, double b) { return a > b ? a : b; }
+
+/* ---- comp.f90:45-85  pressure(q, press): may rewrite q(5) or q(6) ---- */
+static int pressure(const ev_params *p, double *q, double *press) {
+  double m2 = (q[1] * q[1]) + (q[2] * q[2]) + (q[3] * q[3]);
+  double kin = 0.5 * m2 / q[0];
+  double ratio = kin / q[4];
+  double pr;
+  if (p->dualenergy) {
+    if (ratio > p->de_thresh1) {
+      pr = q[5] * pow(q[0], p->g1);     /* comp.f90:63 */
+      q[4] = kin + pr * p->g1inv;       /* comp.f90:64 */
+    } else {
+      pr = p->g1 * (q[4] - kin);        /* comp.f90:67 */
+      q[5] = pr / pow(q[0], p->g1);     /* comp.f90:68 */
+    }
+  } else {
+    pr = p->g1 * (q[4] - kin);          /* comp.f90:71 */
+  }
+  *press = pr;
+  if (pr <= 0.0) return EV_ERR_NEG_PRESSURE; /* comp.f90:74-83 */
+  return EV_OK;
+}
+
+/* ---- comp.f90:26-43  q2w ---- */
+int ev_q2w(const ev_params *p, double *q, double *w) {
+  double densinv = 1.0 / q[0];
+  w[0] = q[0];
+  w[1] = q[1] * densinv;
+  w[2] = q[2] * densinv;
+  w[3] = q[3] * densinv;
+  return pressure(p, q, &w[4]);
+}
+
+/* ---- muscl.f90:26-43  primitive ---- */
+int ev_primitive(const ev_params *p, double *w, double *q) {
+  long first_bad = -1;
+  int iz;
+#pragma omp parallel for schedule(static)
+  for (iz = 1; iz <= p->nz; iz++) {
+    for (int iy = 1; iy <= p->ny; iy++) {
+      for (int ix = 1; ix <= p->nx; ix++) {
+        int e = ev_q2w(p, &q[qidx(p, ix, iy, iz)], &w[widx(p, ix, iy, iz)]);
+        if (e) {
+          long lin = (long)(ix - 1) + (long)p->nx * ((long)(iy - 1) + (long)p->ny * (iz - 1));
+#pragma omp critical(ev_err)
+          if (first_bad < 0 || lin < first_bad) first_bad = lin;
+        }
+      }
+    }
+  }
+  if (first_bad >= 0) {
+    int ix = (int)(first_bad % p->nx) + 1;
+    int iy = (int)((first_bad / p->nx) % p->ny) + 1;
+    int izz = (int)(first_bad / ((long)p->nx * p->ny)) + 1;
+    set_err_cell(ix, iy, izz);
+    return EV_ERR_NEG_PRESSURE;
+  }
+  return EV_OK;
+}
+
+/* ---- boundary.f90:26-113  bound ---- */
+static inline void wcopy(double *dst, const double *src) {
+  for (int c = 0; c < NW; c++) dst[c] = src[c];
+}
+static inline void wcopy_reflect(double *dst, const double *src) {
+  dst[0] = src[0];
+  dst[1] = -src[1];
+  dst[2] = -src[2];
+  dst[3] = -src[3];
+  dst[4] = src[4];
+}
+
+int ev_bound(const ev_params *p, double *w) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz, bt = p->boundtype;
+  /* x boundary over the full y,z extents (boundary.f90:33-66) */
+  if (nx != 1) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++) {
+        if (bt == 1 || bt == 3) {
+          wcopy(&w[widx(p, -1, j, k)], &w[widx(p, 2, j, k)]);
+          wcopy(&w[widx(p, 0, j, k)], &w[widx(p, 1, j, k)]);
+        } else if (bt == 2) {
+          wcopy(&w[widx(p, -1, j, k)], &w[widx(p, nx - 1, j, k)]);
+          wcopy(&w[widx(p, 0, j, k)], &w[widx(p, nx, j, k)]);
+        }
+        if (bt == 1) {
+          wcopy(&w[widx(p, nx + 2, j, k)], &w[widx(p, nx - 1, j, k)]);
+          wcopy(&w[widx(p, nx + 1, j, k)], &w[widx(p, nx, j, k)]);
+        } else if (bt == 2) {
+          wcopy(&w[widx(p, nx + 2, j, k)], &w[widx(p, 2, j, k)]);
+          wcopy(&w[widx(p, nx + 1, j, k)], &w[widx(p, 1, j, k)]);
+        } else if (bt == 3) {
+          wcopy_reflect(&w[widx(p, nx + 1, j, k)], &w[widx(p, nx, j, k)]);
+          wcopy_reflect(&w[widx(p, nx + 2, j, k)], &w[widx(p, nx - 1, j, k)]);
+        }
+      }
+  }
+  /* y boundary over the full x,z extents (boundary.f90:69-87) */
+  if (ny != 1) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++) {
+        if (bt == 1 || bt == 3) {
+          wcopy(&w[widx(p, i, -1, k)], &w[widx(p, i, 2, k)]);
+          wcopy(&w[widx(p, i, 0, k)], &w[widx(p, i, 1, k)]);
+          wcopy(&w[widx(p, i, ny + 2, k)], &w[widx(p, i, ny - 1, k)]);
+          wcopy(&w[widx(p, i, ny + 1, k)], &w[widx(p, i, ny, k)]);
+        } else if (bt == 2) {
+          wcopy(&w[widx(p, i, -1, k)], &w[widx(p, i, ny - 1, k)]);
+          wcopy(&w[widx(p, i, 0, k)], &w[widx(p, i, ny, k)]);
+          wcopy(&w[widx(p, i, ny + 2, k)], &w[widx(p, i, 2, k)]);
+          wcopy(&w[widx(p, i, ny + 1, k)], &w[widx(p, i, 1, k)]);
+        }
+      }
+  }
+  /* z boundary over the full x,y extents (boundary.f90:90-110), one rank */
+  if (nz != 1) {
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) {
+        if (bt == 1 || bt == 3) {
+          wcopy(&w[widx(p, i, j, -1)], &w[widx(p, i, j, 2)]);
+          wcopy(&w[widx(p, i, j, 0)], &w[widx(p, i, j, 1)]);
+          wcopy(&w[widx(p, i, j, nz + 2)], &w[widx(p, i, j, nz - 1)]);
+          wcopy(&w[widx(p, i, j, nz + 1)], &w[widx(p, i, j, nz)]);
+        } else if (bt == 2) {
+          wcopy(&w[widx(p, i, j, -1)], &w[widx(p, i, j, nz - 1)]);
+          wcopy(&w[widx(p, i, j, 0)], &w[widx(p, i, j, nz)]);
+          wcopy(&w[widx(p, i, j, nz + 2)], &w[widx(p, i, j, 2)]);
+          wcopy(&w[widx(p, i, j, nz + 1)], &w[widx(p, i, j, 1)]);
+        }
+      }
+  }
+  return EV_OK;
+}
+
+/* ---- timestep.f90:26-57  hydrostep: dt = min(cfl * v, dt) ---- */
+int ev_hydrostep(const ev_params *p, const double *w, double *dt) {
+  double v = 1e6;
+  int iz;
+#pragma omp parallel for reduction(min : v) schedule(static)
+  for (iz = 1; iz <= p->nz; iz++)
+    for (int iy = 1; iy <= p->ny; iy++)
+      for (int ix = 1; ix <= p->nx; ix++) {
+        const double *c = &w[widx(p, ix, iy, iz)];
+        double a = sqrt(p->g * c[4] / c[0]);
+        double vx = p->dx / (fabs(c[1]) + a);
+        double vy = p->dy / (fabs(c[2]) + a);
+        double vz = p->dz / (fabs(c[3]) + a);
+        v = fmin2(fmin2(fmin2(v, vx), vy), vz);
+      }
+  *dt = fmin2(p->cfl * v, *dt);
+  if (*dt <= 0.0) return EV_ERR_DT;
+  return EV_OK;
+}
+
+/* ---- muscl.f90:115-169  comp_slope ---- */
+int ev_comp_slope(const ev_params *p, const double *wl, const double *w, const double *wr, double *s) {
+  double dwl[NW], dwr[NW];
+  for (int i = 0; i < NW; i++) {
+    dwl[i] = w[i] - wl[i];
+    dwr[i] = wr[i] - w[i];
+  }
+  if (p->limiter == 1) { /* minmod */
+    for (int i = 0; i < NW; i++) {
+      int lpos = dwl[i] >= 0.0, rpos = dwr[i] >= 0.0;
+      if (lpos != rpos) s[i] = 0.0;
+      else if (rpos) s[i] = fmin2(dwl[i], dwr[i]);
+      else s[i] = fmax2(dwl[i], dwr[i]);
+    }
+  } else if (p->limiter == 2) { /* monotonized central */
+    for (int i = 0; i < NW; i++) {
+      double dwc = (wr[i] - wl[i]) * 0.5;
+      int lpos = dwl[i] >= 0.0, rpos = dwr[i] >= 0.0;
+      if (lpos != rpos) s[i] = 0.0;
+      else if (rpos) s[i] = fmin2(fmin2(2.0 * dwl[i], dwc), 2.0 * dwr[i]);
+      else s[i] = fmax2(fmax2(2.0 * dwl[i], dwc), 2.0 * dwr[i]);
+    }
+  } else if (p->limiter == 3) { /* van Leer */
+    for (int i = 0; i < NW; i++) {
+      double pro = dwl[i] * dwr[i];
+      if (pro <= 0.0) s[i] = 0.0;
+      else s[i] = 2.0 * pro / (dwr[i] + dwl[i]);
+    }
+  } else {
+    return EV_ERR_LIMITER;
+  }
+  return EV_OK;
+}
+
+/* predictor contribution of direction d (muscl.f90:100-104), a[] += nothing */
+static inline void predictor_dir(const ev_params *p, const double *c, const double *sd, int d, double *a) {
+  for (int i = 0; i < NW; i++) a[i] = c[d + 1] * sd[i];
+  a[4] = a[4] + p->g * c[4] * sd[d + 1];
+  a[d + 1] = a[d + 1] + sd[4] / c[0];
+}
+
+/* ---- muscl.f90:45-113  compevo ---- */
+int ev_compevo(const ev_params *p, const double *w, double *evo, double *s, double dt) {
+  if (p->limiter < 1 || p->limiter > 3) return EV_ERR_LIMITER;
+  const double dt2 = -0.5 * dt;
+  const long sx = NW, sy = (long)NW * (p->nx + 4), sz = (long)NW * (p->nx + 4) * (p->ny + 4);
+  if (p->onedimension) {
+    for (int ix = 0; ix <= p->nx + 1; ix++) {
+      const double *c = &w[widx(p, ix, 1, 1)];
+      double *sc = &s[(long)3 * NW * eidx(p, ix, 1, 1)];
+      double a[NW];
+      ev_comp_slope(p, c - sx, c, c + sx, sc);
+      predictor_dir(p, c, sc, 0, a);
+      double *e = &evo[(long)NW * eidx(p, ix, 1, 1)];
+      for (int i = 0; i < NW; i++) e[i] = dt2 * a[i] * p->dxinv; /* muscl.f90:68 */
+    }
+  } else if (p->twodimension) {
+    for (int iy = 0; iy <= p->ny + 1; iy++)
+      for (int ix = 0; ix <= p->nx + 1; ix++) {
+        const double *c = &w[widx(p, ix, iy, 1)];
+        double *sc = &s[(long)3 * NW * eidx(p, ix, iy, 1)];
+        double a1[NW], a2[NW];
+        ev_comp_slope(p, c - sx, c, c + sx, sc);
+        ev_comp_slope(p, c - sy, c, c + sy, sc + NW);
+        predictor_dir(p, c, sc, 0, a1);
+        predictor_dir(p, c, sc + NW, 1, a2);
+        double *e = &evo[(long)NW * eidx(p, ix, iy, 1)];
+        for (int i = 0; i < NW; i++) e[i] = dt2 * (a1[i] * p->dxinv + a2[i] * p->dyinv); /* muscl.f90:86 */
+      }
+  } else {
+    int iz;
+#pragma omp parallel for schedule(static)
+    for (iz = 0; iz <= p->nz + 1; iz++)
+      for (int iy = 0; iy <= p->ny + 1; iy++)
+        for (int ix = 0; ix <= p->nx + 1; ix++) {
+          const double *c = &w[widx(p, ix, iy, iz)];
+          double *sc = &s[(long)3 * NW * eidx(p, ix, iy, iz)];
+          double a1[NW], a2[NW], a3[NW];
+          ev_comp_slope(p, c - sx, c, c + sx, sc);
+          ev_comp_slope(p, c - sy, c, c + sy, sc + NW);
+          ev_comp_slope(p, c - sz, c, c + sz, sc + 2 * NW);
+          predictor_dir(p, c, sc, 0, a1);
+          predictor_dir(p, c, sc + NW, 1, a2);
+          predictor_dir(p, c, sc + 2 * NW, 2, a3);
+          double *e = &evo[(long)NW * eidx(p, ix, iy, iz)];
+          for (int i = 0; i < NW; i++)
+            e[i] = dt2 * (a1[i] * p->dxinv + a2[i] * p->dyinv + a3[i] * p->dzinv); /* muscl.f90:106-107 */
+        }
+  }
+  return EV_OK;
+}
+
+/* ---- riemann.f90:157-205  flux ---- */
+void ev_flux(const ev_params *p, const double *w, double *f, int dir) {
+  double u2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+  double ud = w[dir];
+  f[0] = w[0] * ud;
+  for (int i = 1; i <= 3; i++) {
+    if (i == dir) f[i] = w[0] * (ud * ud) + w[4];
+    else f[i] = w[0] * ud * w[i];
+  }
+  f[4] = ud * (0.5 * w[0] * u2 + p->gg1 * w[4]);
+  f[5] = ud * w[4] / pow(w[0], p->g1);
+}
+
+/* ---- riemann.f90:207-239  state ---- */
+void ev_state(const ev_params *p, const double *w, double *q) {
+  q[0] = w[0];
+  q[1] = w[0] * w[1];
+  q[2] = w[0] * w[2];
+  q[3] = w[0] * w[3];
+  double u2 = (w[1] * w[1]) + (w[2] * w[2]) + (w[3] * w[3]);
+  q[4] = 0.5 * w[0] * u2 + w[4] / p->g1;
+  q[5] = w[4] / pow(w[0], p->g1);
+}
+
+/* ---- riemann.f90:241-288  statex (HLLC star state) ---- */
+void ev_statex(const ev_params *p, const double *w, double *qx, double s, double sm, int dir) {
+  double ud = w[dir];
+  qx[0] = w[0] * (s - ud) / (s - sm);
+  for (int i = 1; i <= 3; i++) qx[i] = (i == dir ? sm : w[i]) * qx[0];
+  double energy = 0.5 * w[0] * ((w[1] * w[1]) + (w[2] * w[2]) + (w[3] * w[3])) + w[4] / p->g1;
+  qx[4] = qx[0] * (energy / w[0] + (sm - ud) * (sm + (w[4] / (w[0] * (s - ud)))));
+  qx[5] = qx[0] * w[4] / pow(w[0], p->g);
+}
+
+/* ---- riemann.f90:47-92  hll ---- */
+static int hll(const ev_params *p, const double *wl, const double *wr, double *f, int dir) {
+  if (wl[4] <= 0 || wr[4] <= 0) return EV_ERR_RIEMANN_PRESSURE;
+  double al = sqrt(p->g * wl[4] / wl[0]);
+  double ar = sqrt(p->g * wr[4] / wr[0]);
+  double sl = fmin2(wl[dir] - al, wr[dir] - ar);
+  double sr = fmax2(wl[dir] + al, wr[dir] + ar);
+  if (0 < sl) {
+    ev_flux(p, wl, f, dir);
+  } else if (0 > sr) {
+    ev_flux(p, wr, f, dir);
+  } else if (sl <= 0 && 0 <= sr) {
+    double fl[NQ], fr[NQ], ql[NQ], qr[NQ];
+    ev_flux(p, wl, fl, dir);
+    ev_flux(p, wr, fr, dir);
+    ev_state(p, wl, ql);
+    ev_state(p, wr, qr);
+    for (int i = 0; i < NQ; i++) f[i] = (sr * fl[i] - sl * fr[i] + sl * sr * (qr[i] - ql[i])) / (sr - sl);
+  } else {
+    return EV_ERR_HLL;
+  }
+  return EV_OK;
+}
+
+/* ---- riemann.f90:96-153  hllc ---- */
+static int hllc(const ev_params *p, const double *wl, const double *wr, double *f, int dir) {
+  if (wl[4] <= 0 || wr[4] <= 0) return EV_ERR_RIEMANN_PRESSURE;
+  double al = sqrt(p->g * wl[4] / wl[0]);
+  double ar = sqrt(p->g * wr[4] / wr[0]);
+  double sl = fmin2(wl[dir] - al, wr[dir] - ar);
+  double sr = fmax2(wl[dir] + al, wr[dir] + ar);
+  double facl = wl[0] * (sl - wl[dir]);
+  double facr = wr[0] * (sr - wr[dir]);
+  double sm = (wr[4] - wl[4] + facl * wl[dir] - facr * wr[dir]) / (facl - facr);
+  if (0 < sl) {
+    ev_flux(p, wl, f, dir);
+  } else if (0 > sr) {
+    ev_flux(p, wr, f, dir);
+  } else if (sl < 0 && 0 < sm) {
+    double fl[NQ], ql[NQ], qx[NQ];
+    ev_flux(p, wl, fl, dir);
+    ev_state(p, wl, ql);
+    ev_statex(p, wl, qx, sl, sm, dir);
+    for (int i = 0; i < NQ; i++) f[i] = fl[i] + sl * (qx[i] - ql[i]);
+  } else if (sm <= 0 && 0 < sr) {
+    double fr[NQ], qr[NQ], qx[NQ];
+    ev_flux(p, wr, fr, dir);
+    ev_state(p, wr, qr);
+    ev_statex(p, wr, qx, sr, sm, dir);
+    for (int i = 0; i < NQ; i++) f[i] = fr[i] + sr * (qx[i] - qr[i]);
+  } else {
+    return EV_ERR_HLLC;
+  }
+  return EV_OK;
+}
+
+/* ---- riemann.f90:26-43  riemannsolver; dir 1..3 ---- */
+int ev_riemannsolver(const ev_params *p, const double *wl, const double *wr, double *f, int dir) {
+  if (p->solver == 1) return hll(p, wl, wr, f, dir);
+  if (p->solver == 2) return hllc(p, wl, wr, f, dir);
+  return EV_ERR_SOLVER;
+}
+
+/* face states of muscl.f90:207-210 */
+static inline void face_states(const ev_params *p, const double *w0, const double *w1, const double *s0,
+                               const double *s1, const double *e0, const double *e1, double *wl, double *wr) {
+  for (int c = 0; c < NW; c++) {
+    wl[c] = w0[c];
+    wr[c] = w1[c];
+  }
+  if (p->secondorder) {
+    for (int c = 0; c < NW; c++) {
+      wl[c] = wl[c] + 0.5 * s0[c] + e0[c];
+      wr[c] = wr[c] - 0.5 * s1[c] + e1[c];
+    }
+  }
+}
+
+/* one pencil of sweep (muscl.f90:191-215 and the y/z copies): cells
+ * i = 1..n along `dir`, with w strides ws, evo stride es, s stride es*3*NW. */
+static int pencil(const ev_params *p, const double *w, double *q, const double *evo, const double *s,
+                  long w0, long wst, long e0, long est, long q0, long qst, int n, int dir, double delta,
+                  long *bad_face) {
+  double wl[NW], wr[NW], fl[NQ], fr[NQ];
+  const int sd = (dir - 1) * NW;
+  /* first face: cells 0 and 1 */
+  face_states(p, &w[w0], &w[w0 + wst], &s[3 * NW * e0 + sd], &s[3 * NW * (e0 + est) + sd],
+              &evo[NW * e0], &evo[NW * (e0 + est)], wl, wr);
+  int err = ev_riemannsolver(p, wl, wr, fr, dir);
+  if (err) {
+    *bad_face = 0;
+    return err;
+  }
+  for (int i = 1; i <= n; i++) {
+    for (int c = 0; c < NQ; c++) fl[c] = fr[c];
+    long wi = w0 + i * wst, ei = e0 + i * est;
+    face_states(p, &w[wi], &w[wi + wst], &s[3 * NW * ei + sd], &s[3 * NW * (ei + est) + sd],
+                &evo[NW * ei], &evo[NW * (ei + est)], wl, wr);
+    err = ev_riemannsolver(p, wl, wr, fr, dir);
+    if (err) {
+      *bad_face = i;
+      return err;
+    }
+    double *qc = &q[q0 + (i - 1) * qst];
+    for (int c = 0; c < NQ; c++) qc[c] = qc[c] + delta * (fl[c] - fr[c]); /* muscl.f90:287 */
+  }
+  return EV_OK;
+}
+
+/* ---- muscl.f90:171-274  sweep (x, then y, then z) ---- */
+int ev_sweep(const ev_params *p, const double *w, double *q, const double *evo, const double *s, double dt) {
+  if (p->solver != 1 && p->solver != 2) return EV_ERR_SOLVER;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const long wsx = NW, wsy = (long)NW * (nx + 4), wsz = (long)NW * (nx + 4) * (ny + 4);
+  const long esx = 1, esy = nx + 2, esz = (long)(nx + 2) * (ny + 2);
+  const long qsx = NQ, qsy = (long)NQ * nx, qsz = (long)NQ * nx * ny;
+  int err = EV_OK;
+  if (nx != 1) {
+    double delta = dt * p->dxinv;
+    int iz;
+#pragma omp parallel for schedule(static)
+    for (iz = 1; iz <= nz; iz++)
+      for (int iy = 1; iy <= ny; iy++) {
+        long bf;
+        int e = pencil(p, w, q, evo, s, widx(p, 0, iy, iz), wsx, eidx(p, 0, iy, iz), esx, qidx(p, 1, iy, iz), qsx,
+                       nx, 1, delta, &bf);
+        if (e) {
+#pragma omp critical(ev_err)
+          {
+            err = e;
+            set_err_cell((int)bf, iy, iz);
+          }
+        }
+      }
+    if (err) return err;
+  }
+  if (ny != 1) {
+    double delta = dt * p->dyinv;
+    int iz;
+#pragma omp parallel for schedule(static)
+    for (iz = 1; iz <= nz; iz++)
+      for (int ix = 1; ix <= nx; ix++) {
+        long bf;
+        int e = pencil(p, w, q, evo, s, widx(p, ix, 0, iz), wsy, eidx(p, ix, 0, iz), esy, qidx(p, ix, 1, iz), qsy,
+                       ny, 2, delta, &bf);
+        if (e) {
+#pragma omp critical(ev_err)
+          {
+            err = e;
+            set_err_cell(ix, (int)bf, iz);
+          }
+        }
+      }
+    if (err) return err;
+  }
+  if (nz != 1) {
+    double delta = dt * p->dzinv;
+    int iy;
+#pragma omp parallel for schedule(static)
+    for (iy = 1; iy <= ny; iy++)
+      for (int ix = 1; ix <= nx; ix++) {
+        long bf;
+        int e = pencil(p, w, q, evo, s, widx(p, ix, iy, 0), wsz, eidx(p, ix, iy, 0), esz, qidx(p, ix, iy, 1), qsz,
+                       nz, 3, delta, &bf);
+        if (e) {
+#pragma omp critical(ev_err)
+          {
+            err = e;
+            set_err_cell(ix, iy, (int)bf);
+          }
+        }
+      }
+  }
+  return err;
+}
+
+/* ---- main.f90:109-155 for pure hydro, with updatetime (timestep.f90:163-169) ---- */
+int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, double *t, double tend,
+            double *dt, int *go) {
+  int err = ev_primitive(p, w, q);
+  if (err) return err;
+  ev_bound(p, w);
+  *dt = 1e99;
+  err = ev_hydrostep(p, w, dt);
+  if (err) return err;
+  if (tend - *t <= *dt) {
+    *dt = tend - *t;
+    *go = 0;
+  }
+  *t = *t + *dt;
+  if (p->secondorder) {
+    err = ev_compevo(p, w, evo, s, *dt);
+    if (err) return err;
+  }
+  return ev_sweep(p, w, q, evo, s, *dt);
+}
+
+int ev_run(const ev_params *p, double *q, int nsteps, double *t, double tend, double *dt_last, int *done) {
+  double *w = calloc((size_t)ev_size_w(p), sizeof(double));
+  double *evo = calloc((size_t)ev_size_evo(p), sizeof(double));
+  double *s = calloc((size_t)ev_size_s(p), sizeof(double));
+  int err = EV_OK, go = 1;
+  *done = 0;
+  if (!w || !evo || !s) {
+    err = EV_ERR_CONFIG;
+  } else {
+    for (int k = 0; k < nsteps && go; k++) {
+      err = ev_step(p, q, w, evo, s, t, tend, dt_last, &go);
+      if (err) break;
+      *done = k + 1;
+    }
+  }
+  free(w);
+  free(evo);
+  free(s);
+  return err;
+}
+
+/* ---- snapshot.f90:169-172 sums, without the `volume` factor ---- */
+void ev_sums(const ev_params *p, const double *q, double out[3]) {
+  double m = 0, e = 0, s = 0;
+  long n = (long)p->nx * p->ny * p->nz;
+  for (long i = 0; i < n; i++) {
+    m += q[NQ * i + 0];
+    e += q[NQ * i + 4];
+    s += q[NQ * i + 5];
+  }
+  out[0] = m;
+  out[1] = e;
+  out[2] = s;
+}

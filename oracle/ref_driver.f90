@@ -1,0 +1,133 @@
+!-----------------------------------------------------------------------------
+! ref_driver.f90 -- test harness around the reference's own hot-path objects
+! (TEST INFRASTRUCTURE: oracle/Makefile links it with global, comp, muscl,
+! riemann, boundary, timestep and parameter.f90 compiled in place from the
+! reference tree into oracle/_ref/).
+!
+! It replaces only main.f90's role for pure hydro on one rank:
+!   parameters  readparameter + compparameter   (parameter.f90:26-96,164-265)
+!   ranges      lx=1..nx etc. as decomposition does for mpisize=1
+!               (parallel.f90:33-64)
+!   arrays      q, w, evo, s allocated and zeroed as main.f90:72-77
+!   loop        primitive, bound, dt=1d99, hydrostep, updatetime, compevo,
+!               sweep                                  (main.f90:109-155)
+!
+! usage: evref <parafile> <mode> <in.bin> <out.bin> [nsteps]
+!   mode run    : in = q(nq,nx,ny,nz) f64; nsteps passes; out = q
+!   mode stage  : in = q; one pass; out = w, evo, s, q_after_sweep, dt
+!   mode riemann: in = n records of (wl(5), wr(5), dir); out = f(6) per record
+!   mode slope  : in = n records of (wl(5), w(5), wr(5)); out = s(5) per record
+! Each step prints "STEP k t dt" to stdout.  Reference errors print their own
+! "ERROR ..." lines and `stop` (exit status 0), exactly as in the reference.
+!-----------------------------------------------------------------------------
+program evref
+  use global
+  implicit none
+
+  character(len=256) :: mode, fin, fout, arg
+  real(8), allocatable :: q(:,:,:,:), w(:,:,:,:), evo(:,:,:,:), s(:,:,:,:,:)
+  real(8), allocatable :: rec(:)
+  real(8) :: wl(nw), wr(nw), wc(nw), f(nf), sl(nw)
+  integer :: nsteps, k, n, i, u, fsize, dir, k4
+  logical :: go, dump
+
+  call get_command_argument(2, mode)
+  call get_command_argument(3, fin)
+  call get_command_argument(4, fout)
+  nsteps = 1
+  if (command_argument_count() .ge. 5) then
+     call get_command_argument(5, arg)
+     read (arg, *) nsteps
+  endif
+
+  mpirank = 0 ; mpisize = 1
+  master = .true. ; notmaster = .false. ; last = .true. ; notlast = .false.
+  even = .true. ; noteven = .false. ; parallel = .false.
+
+  call readparameter()
+  call compparameter()
+
+  lx  = 1      ; mx  = nx
+  lx1 = lx - 1 ; lx2 = lx - 2 ; lx0 = lx + 1
+  mx1 = mx + 1 ; mx2 = mx + 2 ; mx0 = mx - 1
+  ly  = 1      ; my  = ny
+  ly1 = ly - 1 ; ly2 = ly - 2 ; ly0 = ly + 1
+  my1 = my + 1 ; my2 = my + 2 ; my0 = my - 1
+  lz  = 1      ; mz  = nz
+  lz1 = lz - 1 ; lz2 = lz - 2 ; lz0 = lz + 1
+  mz1 = mz + 1 ; mz2 = mz + 2 ; mz0 = mz - 1
+
+  if (trim(mode) .eq. 'riemann' .or. trim(mode) .eq. 'slope') then
+     inquire (file=trim(fin), size=fsize)
+     n = fsize / 8
+     allocate(rec(n))
+     open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+     read (u) rec
+     close (u)
+     open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+     if (trim(mode) .eq. 'riemann') then
+        do i = 0, n / 11 - 1
+           wl = rec(11*i+1:11*i+5)
+           wr = rec(11*i+6:11*i+10)
+           dir = nint(rec(11*i+11))
+           call riemannsolver(wl, wr, f, dir)
+           write (u) f
+        enddo
+     else
+        do i = 0, n / 15 - 1
+           wl = rec(15*i+1:15*i+5)
+           wc = rec(15*i+6:15*i+10)
+           wr = rec(15*i+11:15*i+15)
+           call comp_slope(wl, wc, wr, sl)
+           write (u) sl
+        enddo
+     endif
+     close (u)
+     stop
+  endif
+
+  allocate(q(nq,lx:mx,ly:my,lz:mz))
+  allocate(w(nw,lx2:mx2,ly2:my2,lz2:mz2))
+  allocate(evo(nw,lx1:mx1,ly1:my1,lz1:mz1))
+  allocate(s(nw,3,lx1:mx1,ly1:my1,lz1:mz1))
+  w = 0.0
+  evo = 0.0
+  s = 0.0
+
+  open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read (u) q
+  close (u)
+
+  go = .true.
+  dump = .false.
+  k4 = 1
+  if (trim(mode) .eq. 'stage') nsteps = 1
+  do k = 1, nsteps
+     if (.not. go) exit
+     call primitive(w, q)
+     call bound(w)
+     dt = 1d99
+     call hydrostep(w)
+     call updatetime(go, dump, k4)
+     if (secondorder) call compevo(w, evo, s)
+     if (trim(mode) .eq. 'stage') then
+        open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+        write (u) w
+        write (u) evo
+        write (u) s
+     endif
+     call sweep(w, q, evo, s)
+     write (*, '(A,I6,2ES25.16)') 'STEP ', k, t, dt
+  enddo
+
+  if (trim(mode) .eq. 'stage') then
+     write (u) q
+     write (u) dt
+     close (u)
+  else
+     open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+     write (u) q
+     close (u)
+  endif
+
+end program evref
