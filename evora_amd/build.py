@@ -1,0 +1,63 @@
+"""Build libevora_hip.so (gfx950) in-tree with hipcc.
+
+    python -m evora_amd.build          # builds evora_amd/libevora_hip.so
+
+Every translation unit is compiled for gfx950 only.  The staged and fused
+kernels are compiled without floating-point contraction so that their FP64
+results follow the reference's operation order; packed FP32 instructions are
+disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libevora_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+COMMON = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(HERE, "..", "include"),
+]
+
+# source -> extra flags
+UNITS = {
+    "api.hip": [],
+    "staged.hip": ["-ffp-contract=off"],
+    "fused.hip": ["-ffp-contract=off"],
+}
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {os.path.basename(cmd[-1])}")
+    return r
+
+
+def build_library(verbose: bool = False) -> str:
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src, extra in UNITS.items():
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *COMMON, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, OUT)
+    shutil.rmtree(objdir, ignore_errors=True)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build_library(verbose="-v" in sys.argv))

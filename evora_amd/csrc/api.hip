@@ -1,0 +1,573 @@
+// api.hip -- the C-ABI of libevora_hip (include/evora_hip.h).
+//
+// Host side of the drop-in: owns the device state of one grid, maps the
+// reference's step-level calls (primitive, bound, hydrostep, compevo, sweep)
+// onto either the staged kernels or the fused one-pass kernel, and converts
+// between the host AoS layout q(6,nx,ny,nz) and the device SoA layout.
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "evh_internal.h"
+
+using namespace evh;
+
+namespace {
+
+constexpr double kVInit = 1e6;  // timestep.f90:35
+
+const char *err_text(int code) {
+  switch (code) {
+    case EVH_OK: return "ok";
+    case EVH_ERR_NEG_PRESSURE: return "ERROR negative pressure occured while computing primitive variables!";
+    case EVH_ERR_RIEMANN_PRESSURE: return "ERROR: riemann solver unphysical pressure";
+    case EVH_ERR_HLL: return "ERROR: hll solver";
+    case EVH_ERR_HLLC: return "ERROR: hllc solver";
+    case EVH_ERR_DT: return "ERROR: hydrostep is negative";
+    case EVH_ERR_SOLVER: return "ERROR choose proper solver!";
+    case EVH_ERR_LIMITER: return "ERROR: wrong slope limiter!";
+    case EVH_ERR_CONFIG: return "ERROR: unsupported configuration";
+    case EVH_ERR_HIP: return "ERROR: HIP runtime failure";
+    case EVH_ERR_STATE: return "ERROR: call out of order";
+    default: return "ERROR: unknown";
+  }
+}
+
+int fail(evh_ctx *h, int code, const char *detail = nullptr, long long cell = -1) {
+  if (!h) return code;
+  h->err_code = code;
+  h->err_msg = err_text(code);
+  if (detail) {
+    h->err_msg += " (";
+    h->err_msg += detail;
+    h->err_msg += ")";
+  }
+  if (cell >= 0) {
+    const long long nx = h->grid.nx, ny = h->grid.ny;
+    h->err_cell[0] = (int)(cell % nx) + 1;
+    h->err_cell[1] = (int)((cell / nx) % ny) + 1;
+    h->err_cell[2] = (int)(cell / (nx * ny)) + 1;
+  } else {
+    h->err_cell[0] = h->err_cell[1] = h->err_cell[2] = 0;
+  }
+  return code;
+}
+
+int hip_fail(evh_ctx *h, hipError_t e) { return fail(h, EVH_ERR_HIP, hipGetErrorString(e)); }
+
+#define HIP_TRY(x)                         \
+  do {                                     \
+    hipError_t e__ = (x);                  \
+    if (e__ != hipSuccess) return hip_fail(h, e__); \
+  } while (0)
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_reset_status(Status *st, int keep_next) {
+  if (threadIdx.x == 0) {
+    st->vmin_bits = (unsigned long long)__double_as_longlong(kVInit);
+    if (!keep_next) {
+      st->next_code = 0;
+      st->next_cell = 0;
+    }
+  }
+}
+
+__global__ void k_clear_code(Status *st) {
+  if (threadIdx.x == 0) {
+    st->code = 0;
+    st->cell = 0;
+    st->next_code = 0;
+    st->next_cell = 0;
+    st->vmin_bits = (unsigned long long)__double_as_longlong(kVInit);
+  }
+}
+
+// AoS chunk (cells x 6, double) -> SoA storage at cell offset `base`
+template <typename S>
+__global__ void k_aos_to_soa(const double *__restrict__ src, S *__restrict__ dst, long long cells, long long base,
+                             long long n) {
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < cells; c += (long long)gridDim.x * 256) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) dst[m * n + base + c] = S(src[6 * c + m]);
+  }
+}
+
+template <typename S>
+__global__ void k_soa_to_aos(const S *__restrict__ src, double *__restrict__ dst, long long cells, long long base,
+                             long long n) {
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < cells; c += (long long)gridDim.x * 256) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) dst[6 * c + m] = double(src[m * n + base + c]);
+  }
+}
+
+// compensated per-block sums of q1, q5, q6 (snapshot.f90:169-172)
+constexpr int kSumBlocks = 1024;
+template <typename S>
+__global__ void __launch_bounds__(256) k_sums(const S *__restrict__ q, long long n, double *partials) {
+  __shared__ double sh[3][256];
+  double s[3] = {0, 0, 0}, comp[3] = {0, 0, 0};
+  const int comps[3] = {0, 4, 5};
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {  // Kahan
+      const double y = double(q[comps[j] * n + c]) - comp[j];
+      const double t = s[j] + y;
+      comp[j] = (t - s[j]) - y;
+      s[j] = t;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) sh[j][threadIdx.x] = s[j] - comp[j];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int j = 0; j < 3; j++) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < 3; j++) partials[3 * blockIdx.x + j] = sh[j][0];
+}
+
+int grid_of(long long work) {
+  long long b = (work + 255) / 256;
+  if (b > 16384) b = 16384;
+  return b < 1 ? 1 : (int)b;
+}
+
+bool mixed(const evh_ctx *h) { return h->cfg.precision == EVH_FP32_MIXED; }
+
+void accumulate_timing(evh_ctx *h) {
+  if (!h->tev_pending) return;
+  float ms = 0.f;
+  if (hipEventSynchronize(h->tev[1]) == hipSuccess && hipEventElapsedTime(&ms, h->tev[0], h->tev[1]) == hipSuccess) {
+    h->kernel_ms += ms;
+    h->kernel_launches += 1;
+  }
+  h->tev_pending = false;
+}
+
+// Sync and surface device-side errors.  Errors stay sticky until read here.
+int check_status(evh_ctx *h) {
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  accumulate_timing(h);
+  HIP_TRY(hipMemcpy(h->st_host, h->st, sizeof(Status), hipMemcpyDeviceToHost));
+  if (h->st_host->code) {
+    const int code = h->st_host->code;
+    const long long cell = (long long)h->st_host->cell;
+    hipLaunchKernelGGL(k_clear_code, dim3(1), dim3(64), 0, h->stream, h->st);
+    hipStreamSynchronize(h->stream);
+    h->v_valid = false;
+    return fail(h, code, "device", cell);
+  }
+  return EVH_OK;
+}
+
+int ensure_staged_arrays(evh_ctx *h) {
+  if (h->w) return EVH_OK;
+  const Grid &G = h->grid;
+  HIP_TRY(hipMalloc(&h->w, sizeof(double) * 5 * G.nw()));
+  HIP_TRY(hipMalloc(&h->evo, sizeof(double) * 5 * G.ne()));
+  HIP_TRY(hipMalloc(&h->s, sizeof(double) * 15 * G.ne()));
+  HIP_TRY(hipMemsetAsync(h->w, 0, sizeof(double) * 5 * G.nw(), h->stream));
+  HIP_TRY(hipMemsetAsync(h->evo, 0, sizeof(double) * 5 * G.ne(), h->stream));  // main.f90:76-77
+  HIP_TRY(hipMemsetAsync(h->s, 0, sizeof(double) * 15 * G.ne(), h->stream));
+  h->dev_bytes += (long long)sizeof(double) * (5 * G.nw() + 20 * G.ne());
+  return EVH_OK;
+}
+
+int validate(const evh_config *c) {
+  if (c->nx < 1 || c->ny < 1 || c->nz < 1) return EVH_ERR_CONFIG;
+  if (c->solver != 1 && c->solver != 2) return EVH_ERR_SOLVER;
+  if (c->limiter < 1 || c->limiter > 3) return EVH_ERR_LIMITER;
+  if (c->boundtype < 1 || c->boundtype > 3) return EVH_ERR_CONFIG;
+  // degenerate dimensions the reference handles: 1-D (ny=nz=1), 2-D (nz=1)
+  const bool oned = c->ny == 1 && c->nz == 1, twod = c->nz == 1;
+  if ((bool)c->onedimension != oned || (bool)c->twodimension != twod) return EVH_ERR_CONFIG;
+  if (c->nx < 2 || (!oned && c->ny < 2)) return EVH_ERR_CONFIG;
+  if (c->precision != EVH_FP64 && c->precision != EVH_FP32_MIXED) return EVH_ERR_CONFIG;
+  if (c->precision == EVH_FP32_MIXED && !fused_supported(*c)) return EVH_ERR_CONFIG;
+  return EVH_OK;
+}
+
+}  // namespace
+
+namespace evh {
+hipError_t launch_reset_status(evh_ctx *h, bool keep_next) {
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, h->stream, h->st, keep_next ? 1 : 0);
+  return hipGetLastError();
+}
+}  // namespace evh
+
+extern "C" {
+
+int evh_create(const evh_config *cfg, evh_ctx **out) {
+  if (!cfg || !out) return EVH_ERR_CONFIG;
+  *out = nullptr;
+  const int v = validate(cfg);
+  if (v) return v;
+  evh_ctx *h = new (std::nothrow) evh_ctx();
+  if (!h) return EVH_ERR_HIP;
+  h->cfg = *cfg;
+  KParams &k = h->kp;
+  k.nx = cfg->nx; k.ny = cfg->ny; k.nz = cfg->nz;
+  k.boundtype = cfg->boundtype; k.solver = cfg->solver; k.limiter = cfg->limiter;
+  k.secondorder = cfg->secondorder; k.dualenergy = cfg->dualenergy;
+  k.onedimension = cfg->onedimension; k.twodimension = cfg->twodimension;
+  k.g = cfg->g; k.g1 = cfg->g1; k.g1inv = cfg->g1inv; k.gg1 = cfg->gg1; k.cfl = cfg->cfl;
+  k.de_thresh1 = cfg->de_thresh1;
+  k.dx = cfg->dx; k.dy = cfg->dy; k.dz = cfg->dz;
+  k.dxinv = cfg->dxinv; k.dyinv = cfg->dyinv; k.dzinv = cfg->dzinv;
+  k.mach_fp64 = cfg->mach_fp64;
+  h->grid = Grid{cfg->nx, cfg->ny, cfg->nz};
+  h->qelem = mixed(h) ? sizeof(float) : sizeof(double);
+  h->mode = fused_supported(*cfg) ? EVH_MODE_FUSED : EVH_MODE_STAGED;
+
+  auto bail = [&](hipError_t e) {
+    int r = hip_fail(h, e);
+    evh_destroy(h);
+    return r;
+  };
+  hipError_t e;
+  if ((e = hipGetDevice(&h->device)) != hipSuccess) return bail(e);
+  if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e);
+  for (auto &ev : h->events)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
+  for (auto &ev : h->tev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
+  const long long n = h->grid.n();
+  for (int b = 0; b < 2; b++) {
+    if ((e = hipMalloc(&h->qbuf[b], h->qelem * 6 * n)) != hipSuccess) return bail(e);
+    h->dev_bytes += (long long)h->qelem * 6 * n;
+  }
+  const long long plane = 6LL * cfg->nx * cfg->ny;
+  long long planes = (32LL << 20) / plane;  // ~256 MB of staging
+  if (planes < 1) planes = 1;
+  if (planes > cfg->nz) planes = cfg->nz;
+  h->stage_elems = (size_t)(planes * plane);
+  if ((e = hipMalloc(&h->stage, sizeof(double) * h->stage_elems)) != hipSuccess) return bail(e);
+  if ((e = hipMalloc(&h->st, sizeof(Status))) != hipSuccess) return bail(e);
+  if ((e = hipHostMalloc(&h->st_host, sizeof(Status))) != hipSuccess) return bail(e);
+  if ((e = hipMalloc(&h->partials, sizeof(double) * 3 * kSumBlocks)) != hipSuccess) return bail(e);
+  h->dev_bytes += (long long)sizeof(double) * h->stage_elems + sizeof(Status) + sizeof(double) * 3 * kSumBlocks;
+  hipLaunchKernelGGL(k_clear_code, dim3(1), dim3(64), 0, h->stream, h->st);
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(e);
+  *out = h;
+  return EVH_OK;
+}
+
+int evh_destroy(evh_ctx *h) {
+  if (!h) return EVH_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto p : h->qbuf) if (p) hipFree(p);
+  if (h->w) hipFree(h->w);
+  if (h->evo) hipFree(h->evo);
+  if (h->s) hipFree(h->s);
+  if (h->stage) hipFree(h->stage);
+  if (h->st) hipFree(h->st);
+  if (h->st_host) hipHostFree(h->st_host);
+  if (h->partials) hipFree(h->partials);
+  for (auto ev : h->events) if (ev) hipEventDestroy(ev);
+  for (auto ev : h->tev) if (ev) hipEventDestroy(ev);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return EVH_OK;
+}
+
+int evh_set_mode(evh_ctx *h, int mode) {
+  if (!h) return EVH_ERR_STATE;
+  if (mode == EVH_MODE_FUSED && !fused_supported(h->cfg)) return fail(h, EVH_ERR_CONFIG, "fused needs a 3-D FP64 grid");
+  if (mode == EVH_MODE_STAGED && mixed(h)) return fail(h, EVH_ERR_CONFIG, "staged kernels are FP64");
+  if (mode != EVH_MODE_FUSED && mode != EVH_MODE_STAGED) return fail(h, EVH_ERR_CONFIG);
+  h->mode = mode;
+  h->v_valid = false;
+  h->w_valid = false;
+  return EVH_OK;
+}
+
+int evh_upload_q(evh_ctx *h, const double *q) {
+  if (!h || !q) return EVH_ERR_STATE;
+  const long long n = h->grid.n();
+  const long long chunk = (long long)h->stage_elems / 6;
+  for (long long base = 0; base < n; base += chunk) {
+    const long long cells = (n - base < chunk) ? n - base : chunk;
+    HIP_TRY(hipMemcpyAsync(h->stage, q + 6 * base, sizeof(double) * 6 * cells, hipMemcpyHostToDevice, h->stream));
+    if (mixed(h))
+      hipLaunchKernelGGL(k_aos_to_soa<float>, dim3(grid_of(cells)), dim3(256), 0, h->stream, h->stage,
+                         (float *)h->qbuf[h->cur], cells, base, n);
+    else
+      hipLaunchKernelGGL(k_aos_to_soa<double>, dim3(grid_of(cells)), dim3(256), 0, h->stream, h->stage,
+                         (double *)h->qbuf[h->cur], cells, base, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  h->v_valid = false;
+  h->w_valid = false;
+  return EVH_OK;
+}
+
+int evh_download_q(evh_ctx *h, double *q) {
+  if (!h || !q) return EVH_ERR_STATE;
+  int r = check_status(h);
+  if (r) return r;
+  const long long n = h->grid.n();
+  const long long chunk = (long long)h->stage_elems / 6;
+  for (long long base = 0; base < n; base += chunk) {
+    const long long cells = (n - base < chunk) ? n - base : chunk;
+    if (mixed(h))
+      hipLaunchKernelGGL(k_soa_to_aos<float>, dim3(grid_of(cells)), dim3(256), 0, h->stream,
+                         (const float *)h->qbuf[h->cur], h->stage, cells, base, n);
+    else
+      hipLaunchKernelGGL(k_soa_to_aos<double>, dim3(grid_of(cells)), dim3(256), 0, h->stream,
+                         (const double *)h->qbuf[h->cur], h->stage, cells, base, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(q + 6 * base, h->stage, sizeof(double) * 6 * cells, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  return EVH_OK;
+}
+
+// ---------------------------------------------------------------- step-level routines
+int evh_primitive(evh_ctx *h) {
+  if (!h) return EVH_ERR_STATE;
+  if (h->mode == EVH_MODE_STAGED) {
+    int r = ensure_staged_arrays(h);
+    if (r) return r;
+    HIP_TRY(launch_primitive(h));
+    h->w_valid = true;
+    h->bound_done = false;
+    return check_status(h);
+  }
+  if (!h->v_valid) {
+    HIP_TRY(launch_reset_status(h, false));
+    HIP_TRY(launch_cfl(h));
+    h->v_valid = true;
+  }
+  return EVH_OK;
+}
+
+int evh_bound(evh_ctx *h) {
+  if (!h) return EVH_ERR_STATE;
+  if (h->mode == EVH_MODE_STAGED) {
+    if (!h->w_valid) return fail(h, EVH_ERR_STATE, "bound before primitive");
+    HIP_TRY(launch_bound(h));
+    h->bound_done = true;
+  }
+  return EVH_OK;
+}
+
+int evh_hydrostep(evh_ctx *h, double *dt) {
+  if (!h || !dt) return EVH_ERR_STATE;
+  if (h->mode == EVH_MODE_STAGED) {
+    if (!h->w_valid) return fail(h, EVH_ERR_STATE, "hydrostep before primitive");
+    HIP_TRY(launch_reset_status(h, false));
+    HIP_TRY(launch_hydrostep(h));
+  } else if (!h->v_valid) {
+    return fail(h, EVH_ERR_STATE, "hydrostep before primitive");
+  }
+  int r = check_status(h);
+  if (r) return r;
+  if (h->st_host->next_code) {
+    h->v_valid = false;
+    return fail(h, (int)h->st_host->next_code, "primitive", (long long)h->st_host->next_cell);
+  }
+  double v;
+  const unsigned long long bits = h->st_host->vmin_bits;
+  std::memcpy(&v, &bits, sizeof v);
+  const double c = h->cfg.cfl * v;
+  *dt = c < *dt ? c : *dt;  // timestep.f90:49
+  if (*dt <= 0.0) return fail(h, EVH_ERR_DT);
+  return EVH_OK;
+}
+
+int evh_compevo(evh_ctx *h, double dt) {
+  if (!h) return EVH_ERR_STATE;
+  if (h->mode == EVH_MODE_STAGED) {
+    if (!h->bound_done) return fail(h, EVH_ERR_STATE, "compevo before bound");
+    HIP_TRY(launch_compevo(h, dt));
+  }
+  h->dt_evo = dt;
+  h->evo_valid = true;
+  return EVH_OK;
+}
+
+int evh_sweep(evh_ctx *h, double dt) {
+  if (!h) return EVH_ERR_STATE;
+  if (h->cfg.secondorder && (!h->evo_valid || h->dt_evo != dt))
+    return fail(h, EVH_ERR_STATE, "sweep without compevo at the same dt");
+  h->evo_valid = false;
+  if (h->mode == EVH_MODE_STAGED) {
+    if (!h->bound_done) return fail(h, EVH_ERR_STATE, "sweep before bound");
+    accumulate_timing(h);
+    if (h->timing) HIP_TRY(hipEventRecord(h->tev[0], h->stream));
+    HIP_TRY(launch_sweep(h, dt));
+    if (h->timing) {
+      HIP_TRY(hipEventRecord(h->tev[1], h->stream));
+      h->tev_pending = true;
+    }
+    h->w_valid = false;
+    h->bound_done = false;
+    return check_status(h);
+  }
+  if (!h->v_valid) return fail(h, EVH_ERR_STATE, "sweep before primitive");
+  accumulate_timing(h);
+  HIP_TRY(launch_reset_status(h, false));
+  if (h->timing) HIP_TRY(hipEventRecord(h->tev[0], h->stream));
+  HIP_TRY(launch_fused_step(h, dt));
+  if (h->timing) {
+    HIP_TRY(hipEventRecord(h->tev[1], h->stream));
+    h->tev_pending = true;
+  }
+  h->cur = 1 - h->cur;
+  h->v_valid = true;
+  return EVH_OK;
+}
+
+int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go) {
+  if (!h || !t || !dt || !go) return EVH_ERR_STATE;
+  int r;
+  if ((r = evh_primitive(h))) return r;
+  if ((r = evh_bound(h))) return r;
+  *dt = 1e99;  // main.f90:118
+  if ((r = evh_hydrostep(h, dt))) return r;
+  if (tend - *t <= *dt) {  // timestep.f90:163-166
+    *dt = tend - *t;
+    *go = 0;
+  }
+  *t = *t + *dt;
+  if (h->cfg.secondorder && (r = evh_compevo(h, *dt))) return r;
+  return evh_sweep(h, *dt);
+}
+
+int evh_run(evh_ctx *h, int nsteps, double *t, double tend, double *dt, int *done) {
+  if (!h || !t || !dt || !done) return EVH_ERR_STATE;
+  int go = 1;
+  *done = 0;
+  for (int k = 0; k < nsteps && go; k++) {
+    int r = evh_step(h, t, tend, dt, &go);
+    if (r) return r;
+    *done = k + 1;
+  }
+  return check_status(h);
+}
+
+int evh_sums(evh_ctx *h, double out[3]) {
+  if (!h || !out) return EVH_ERR_STATE;
+  int r = check_status(h);
+  if (r) return r;
+  const long long n = h->grid.n();
+  if (mixed(h))
+    hipLaunchKernelGGL(k_sums<float>, dim3(kSumBlocks), dim3(256), 0, h->stream, (const float *)h->qbuf[h->cur], n,
+                       h->partials);
+  else
+    hipLaunchKernelGGL(k_sums<double>, dim3(kSumBlocks), dim3(256), 0, h->stream, (const double *)h->qbuf[h->cur], n,
+                       h->partials);
+  HIP_TRY(hipGetLastError());
+  double part[3 * kSumBlocks];
+  HIP_TRY(hipMemcpyAsync(part, h->partials, sizeof part, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int j = 0; j < 3; j++) {
+    double s = 0, c = 0;
+    for (int b = 0; b < kSumBlocks; b++) {
+      const double y = part[3 * b + j] - c;
+      const double t = s + y;
+      c = (t - s) - y;
+      s = t;
+    }
+    out[j] = s;
+  }
+  return EVH_OK;
+}
+
+// ---------------------------------------------------------------- staged-array views
+int evh_download_w(evh_ctx *h, double *w) {
+  if (!h || !w) return EVH_ERR_STATE;
+  if (h->mode != EVH_MODE_STAGED || !h->w) return fail(h, EVH_ERR_STATE, "w exists in staged mode only");
+  int r = check_status(h);
+  if (r) return r;
+  const Grid &G = h->grid;
+  const long long nw = G.nw();
+  double *tmp = new (std::nothrow) double[5 * nw];
+  if (!tmp) return fail(h, EVH_ERR_HIP, "host allocation");
+  hipError_t e = hipMemcpy(tmp, h->w, sizeof(double) * 5 * nw, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (long long c = 0; c < nw; c++)
+      for (int m = 0; m < 5; m++) w[5 * c + m] = tmp[m * nw + c];
+  delete[] tmp;
+  return e == hipSuccess ? EVH_OK : hip_fail(h, e);
+}
+
+int evh_download_evo_s(evh_ctx *h, double *evo, double *s) {
+  if (!h || !evo || !s) return EVH_ERR_STATE;
+  if (h->mode != EVH_MODE_STAGED || !h->evo) return fail(h, EVH_ERR_STATE, "evo/s exist in staged mode only");
+  int r = check_status(h);
+  if (r) return r;
+  const long long ne = h->grid.ne();
+  double *tmp = new (std::nothrow) double[15 * ne];
+  if (!tmp) return fail(h, EVH_ERR_HIP, "host allocation");
+  hipError_t e = hipMemcpy(tmp, h->evo, sizeof(double) * 5 * ne, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (long long c = 0; c < ne; c++)
+      for (int m = 0; m < 5; m++) evo[5 * c + m] = tmp[m * ne + c];
+  if (e == hipSuccess) e = hipMemcpy(tmp, h->s, sizeof(double) * 15 * ne, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (long long c = 0; c < ne; c++)
+      for (int d = 0; d < 3; d++)
+        for (int m = 0; m < 5; m++) s[15 * c + 5 * d + m] = tmp[(m + 5 * d) * ne + c];
+  delete[] tmp;
+  return e == hipSuccess ? EVH_OK : hip_fail(h, e);
+}
+
+int evh_last_error(evh_ctx *h, int *code, int cell[3], char *msg, int msglen) {
+  if (!h) return EVH_ERR_STATE;
+  if (code) *code = h->err_code;
+  if (cell) for (int i = 0; i < 3; i++) cell[i] = h->err_cell[i];
+  if (msg && msglen > 0) {
+    std::strncpy(msg, h->err_msg.c_str(), (size_t)msglen - 1);
+    msg[msglen - 1] = 0;
+  }
+  return EVH_OK;
+}
+
+int evh_event_record(evh_ctx *h, int slot) {
+  if (!h || slot < 0 || slot >= 8) return EVH_ERR_STATE;
+  HIP_TRY(hipEventRecord(h->events[slot], h->stream));
+  return EVH_OK;
+}
+
+int evh_event_elapsed(evh_ctx *h, int a, int b, float *ms) {
+  if (!h || !ms || a < 0 || a >= 8 || b < 0 || b >= 8) return EVH_ERR_STATE;
+  HIP_TRY(hipEventSynchronize(h->events[b]));
+  HIP_TRY(hipEventElapsedTime(ms, h->events[a], h->events[b]));
+  return EVH_OK;
+}
+
+int evh_synchronize(evh_ctx *h) {
+  if (!h) return EVH_ERR_STATE;
+  return check_status(h);
+}
+
+int evh_set_timing(evh_ctx *h, int on) {
+  if (!h) return EVH_ERR_STATE;
+  int r = check_status(h);
+  if (r) return r;
+  h->timing = on != 0;
+  h->kernel_ms = 0.0;
+  h->kernel_launches = 0;
+  return EVH_OK;
+}
+
+int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches) {
+  if (!h || !total_ms || !launches) return EVH_ERR_STATE;
+  int r = check_status(h);
+  if (r) return r;
+  *total_ms = h->kernel_ms;
+  *launches = h->kernel_launches;
+  return EVH_OK;
+}
+
+long long evh_device_bytes(evh_ctx *h) { return h ? h->dev_bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// evh_internal.h -- context and kernel launchers of libevora_hip (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/evora_hip.h"
+#include "physics.h"
+
+namespace evh {
+
+// Device SoA layouts (x fastest):
+//   q   [6][nz][ny][nx]                      storage type per precision
+//   w   [5][nz+4][ny+4][nx+4]  (staged)      ghost cell g at g+2
+//   evo [5][nz+2][ny+2][nx+2]  (staged)      cell c at c (reference index 0..n+1)
+//   s   [15][nz+2][ny+2][nx+2] (staged)      component c of direction d at c + 5*d
+struct Grid {
+  int nx, ny, nz;
+  __host__ __device__ long long n() const { return (long long)nx * ny * nz; }
+  __host__ __device__ long long nw() const { return (long long)(nx + 4) * (ny + 4) * (nz + 4); }
+  __host__ __device__ long long ne() const { return (long long)(nx + 2) * (ny + 2) * (nz + 2); }
+};
+
+}  // namespace evh
+
+struct evh_ctx {
+  evh_config cfg;
+  evh::KParams kp;
+  evh::Grid grid;
+  int device = 0;
+  int mode = EVH_MODE_FUSED;
+  hipStream_t stream = nullptr;
+  hipEvent_t events[8] = {};
+
+  // conserved state: two buffers (fused path ping-pongs), storage precision per cfg.precision
+  void *qbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  size_t qelem = 8;  // bytes per stored value
+
+  // staged path arrays (allocated on first use)
+  double *w = nullptr, *evo = nullptr, *s = nullptr;
+
+  // transfer staging
+  double *stage = nullptr;
+  size_t stage_elems = 0;
+
+  // reductions
+  evh::Status *st = nullptr;      // device
+  evh::Status *st_host = nullptr; // pinned
+  double *partials = nullptr;     // per-block sums
+  int npartials = 0;
+
+  // step state
+  bool v_valid = false;  // st->vmin_bits holds the CFL min of the current q
+  bool w_valid = false;  // staged: w holds primitive of current q (bound applied?)
+  bool bound_done = false;
+  double dt_evo = 0.0;
+  bool evo_valid = false;
+
+  // last error
+  int err_code = 0;
+  int err_cell[3] = {0, 0, 0};
+  std::string err_msg;
+
+  long long dev_bytes = 0;
+
+  // kernel timing (evh_set_timing)
+  bool timing = false;
+  hipEvent_t tev[2] = {nullptr, nullptr};
+  bool tev_pending = false;
+  double kernel_ms = 0.0;
+  long long kernel_launches = 0;
+};
+
+namespace evh {
+
+// staged kernels (staged.hip)
+hipError_t launch_primitive(evh_ctx *h);
+hipError_t launch_bound(evh_ctx *h);
+hipError_t launch_hydrostep(evh_ctx *h);
+hipError_t launch_compevo(evh_ctx *h, double dt);
+hipError_t launch_sweep(evh_ctx *h, double dt);
+
+// fused kernels (fused.hip)
+hipError_t launch_cfl(evh_ctx *h);                 // primitive + CFL min over cur q
+hipError_t launch_fused_step(evh_ctx *h, double dt);  // cur -> other, CFL of the result
+bool fused_supported(const evh_config &c);
+
+// transfers and reductions (api.hip)
+hipError_t launch_reset_status(evh_ctx *h, bool keep_next);
+hipError_t launch_sums(evh_ctx *h, double *out3);
+
+}  // namespace evh

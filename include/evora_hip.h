@@ -1,0 +1,117 @@
+/*
+ * evora_hip.h -- C-ABI of the MI355X hydro step of evora.
+ *
+ * A drop-in for the reference's hydro routines, called once per time step by
+ * the unchanged driver loop (source/main.f90:109-155).  The Fortran binding
+ * is evora_amd/fortran/evora_hip.f90 (iso_c_binding); Python binds the same
+ * symbols with ctypes (evora_amd/hydro.py).  All entry points return an
+ * EVH_* status; none throws.  One context owns one HIP stream and the
+ * device-resident conserved state; contexts are not thread-safe.
+ *
+ * Host arrays use the reference layouts (component fastest, Fortran order):
+ *   q   (6, nx, ny, nz)                 source/main.f90:72
+ *   w   (5, -1:nx+2, -1:ny+2, -1:nz+2)  source/main.f90:73
+ *   evo (5, 0:nx+1, 0:ny+1, 0:nz+1)     source/main.f90:74
+ *   s   (5, 3, 0:nx+1, 0:ny+1, 0:nz+1)  source/main.f90:75
+ */
+#ifndef EVORA_HIP_H
+#define EVORA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; 1-7 are the reference's own error stops */
+enum {
+  EVH_OK = 0,
+  EVH_ERR_NEG_PRESSURE = 1,     /* comp.f90:74-83   'negative pressure occured'      (stop)      */
+  EVH_ERR_RIEMANN_PRESSURE = 2, /* riemann.f90:61-66,111-116 'unphysical pressure'     (stop)      */
+  EVH_ERR_HLL = 3,              /* riemann.f90:87-90  'hll solver'                    (stop)      */
+  EVH_ERR_HLLC = 4,             /* riemann.f90:148-151 'hllc solver'                  (stop)      */
+  EVH_ERR_DT = 5,               /* timestep.f90:51-55 'hydrostep is negative'         (cleanstop) */
+  EVH_ERR_SOLVER = 6,           /* riemann.f90:37-41  'choose proper solver'          (cleanstop) */
+  EVH_ERR_LIMITER = 7,          /* muscl.f90:164-167  'wrong slope limiter'           (stop)      */
+  EVH_ERR_CONFIG = 8,           /* unsupported grid or parameter combination                       */
+  EVH_ERR_HIP = 9,              /* HIP runtime failure (out of memory, no device, launch failure)   */
+  EVH_ERR_STATE = 10            /* call out of order (e.g. sweep before hydrostep)                  */
+};
+
+enum { EVH_FP64 = 0, EVH_FP32_MIXED = 1 };
+enum { EVH_MODE_FUSED = 0, EVH_MODE_STAGED = 1 };
+
+/* Constants of module `global` the step reads (global.f90:44-86), as derived
+ * by compparameter (parameter.f90:164-265).  The host passes them; the
+ * library never recomputes them. */
+typedef struct evh_config {
+  int nx, ny, nz;
+  int boundtype;    /* 1 zero-gradient (mirror), 2 periodic, 3 zero-gradient + reflecting upper x */
+  int solver;       /* 1 HLL, 2 HLLC */
+  int limiter;      /* 1 minmod, 2 monotonized central, 3 van Leer */
+  int secondorder;  /* 0: first order, compevo skipped (main.f90:151) */
+  int dualenergy;
+  int onedimension, twodimension;
+  int precision;    /* EVH_FP64 or EVH_FP32_MIXED */
+  double g, g1, g1inv, gg1, cfl, de_thresh1;
+  double dx, dy, dz, dxinv, dyinv, dzinv;
+  double volume;
+  double mach_fp64; /* FP32_MIXED: pressure from E in FP64 where Mach > mach_fp64 (e.g. 5) */
+} evh_config;
+
+typedef struct evh_ctx evh_ctx;
+
+/* Create a context on the current HIP device; allocates the device state. */
+int evh_create(const evh_config *cfg, evh_ctx **out);
+int evh_destroy(evh_ctx *h);
+
+/* Select the kernel structure: EVH_MODE_FUSED (one HBM pass per step, the
+ * default for 3-D grids) or EVH_MODE_STAGED (one kernel per reference
+ * routine, with w, evo and s materialised in HBM; the path for 1-D/2-D). */
+int evh_set_mode(evh_ctx *h, int mode);
+
+/* Host q (reference layout) <-> device state. */
+int evh_upload_q(evh_ctx *h, const double *q);
+int evh_download_q(evh_ctx *h, double *q);
+
+/* ---- the reference's step-level routines, same order of calls ---- */
+int evh_primitive(evh_ctx *h);                   /* primitive(w,q)     muscl.f90:26-43        */
+int evh_bound(evh_ctx *h);                       /* bound(w)           boundary.f90:26-113    */
+int evh_hydrostep(evh_ctx *h, double *dt);       /* hydrostep(w)       timestep.f90:26-57:
+                                                    dt = min(cfl*v, dt), in/out              */
+int evh_compevo(evh_ctx *h, double dt);          /* compevo(w,evo,s)   muscl.f90:45-113       */
+int evh_sweep(evh_ctx *h, double dt);            /* sweep(w,q,evo,s)   muscl.f90:171-274      */
+
+/* One pass of main.f90:109-155 for pure hydro on the device, with
+ * updatetime (timestep.f90:163-169): t, dt, go in/out.  evh_run repeats it
+ * nsteps times (or until go turns 0), syncing only for dt. */
+int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go);
+int evh_run(evh_ctx *h, int nsteps, double *t, double tend, double *dt, int *done);
+
+/* snapshot.f90:169-172 sums of q1, q5, q6 (without the volume factor), FP64
+ * compensated on the device. */
+int evh_sums(evh_ctx *h, double out[3]);
+
+/* Staged mode only: the last primitive+bound w, and the last compevo evo, s. */
+int evh_download_w(evh_ctx *h, double *w);
+int evh_download_evo_s(evh_ctx *h, double *evo, double *s);
+
+/* Last error: status code, cell (Fortran indices) and a message. */
+int evh_last_error(evh_ctx *h, int *code, int cell[3], char *msg, int msglen);
+
+/* Timing support: events recorded on the context's stream. */
+int evh_event_record(evh_ctx *h, int slot);                 /* slot 0..7 */
+int evh_event_elapsed(evh_ctx *h, int a, int b, float *ms); /* syncs on slot b */
+int evh_synchronize(evh_ctx *h);
+
+/* Kernel timing (no counterpart in the reference; for benchmarks): when on,
+ * every fused or staged sweep launch is bracketed by events on the context's
+ * stream and its duration is added to a running total at the next sync. */
+int evh_set_timing(evh_ctx *h, int on);
+int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches);
+
+/* Bytes of device memory the context holds. */
+long long evh_device_bytes(evh_ctx *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+"""GPU parity of the fused one-pass FP64 step against the CPU oracle.
+
+Bar (north star): relative L-infinity <= 1e-12 per component after one
+step, relative L1 <= 1e-10 after a multi-step run.  rel L-inf of a component
+is max|gpu - oracle| / max|oracle|.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l1, rel_linf
+from evora_amd import hydro, ics
+from evora_amd.params import HydroParams
+
+pytestmark = pytest.mark.gpu
+
+ONE_STEP_TOL = 1e-12
+RUN_L1_TOL = 1e-10
+
+
+def one_step(p, q0):
+    d = p.derive()
+    _, _, _, q1_o, dt_o = O.Oracle(d).stage(q0)
+    with hydro.Hydro(d, mode=hydro.MODE_FUSED) as h:
+        h.upload_q(q0)
+        h.primitive()
+        h.bound()
+        dt = h.hydrostep(1e99)
+        h.compevo(dt)
+        h.sweep(dt)
+        q1 = h.download_q()
+        dt_next = h.hydrostep(1e99)  # CFL folded into the sweep
+    return q1, q1_o, dt, dt_o, dt_next
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("limiter", [1, 2, 3])
+@pytest.mark.parametrize("bt", [1, 2, 3])
+def test_fused_one_step(solver, limiter, bt):
+    p = HydroParams(nx=19, ny=17, nz=11, solver=solver, limiter=limiter, boundtype=bt, cfl=0.4, atomic=True)
+    q0 = ics.random_smooth(p, seed=100 * solver + 10 * limiter + bt)
+    q1, q1_o, dt, dt_o, dt_next = one_step(p, q0)
+    assert dt == dt_o
+    for c in range(6):
+        assert rel_linf(q1[..., c], q1_o[..., c]) <= ONE_STEP_TOL, c
+    # the folded CFL equals a fresh hydrostep on the new state
+    d = p.derive()
+    w = O.new_w(d)
+    qq = q1_o.copy()
+    O.Oracle(d).primitive(w, qq)
+    assert dt_next == pytest.approx(O.Oracle(d).hydrostep(w, 1e99), rel=1e-13)
+
+
+@pytest.mark.parametrize("shape", [(30, 28, 20), (16, 45, 9), (64, 16, 70)])
+def test_fused_tile_edges(shape):
+    """Grids that are not multiples of the 14x14 tile or the z chunk."""
+    nx, ny, nz = shape
+    p = HydroParams(nx=nx, ny=ny, nz=nz, solver=2, limiter=1, boundtype=1, cfl=0.4, atomic=True)
+    q0 = ics.random_smooth(p, seed=nx + ny + nz)
+    q1, q1_o, dt, dt_o, _ = one_step(p, q0)
+    for c in range(6):
+        assert rel_linf(q1[..., c], q1_o[..., c]) <= ONE_STEP_TOL, c
+
+
+@pytest.mark.parametrize("name", ["run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3"])
+def test_fused_run_matches_reference_fixture(name, run_fixtures, golden_index):
+    p = HydroParams(**golden_index[name]["params"])
+    q0 = run_fixtures[name + "/q0"]
+    t_dt = run_fixtures[name + "/t_dt"]
+    with hydro.Hydro(p, mode=hydro.MODE_FUSED) as h:
+        h.upload_q(q0)
+        t, dt, n = h.run(len(t_dt), p.tstart, p.tend)
+        q = h.download_q()
+    assert n == len(t_dt)
+    assert t == pytest.approx(t_dt[-1][0], rel=1e-13)
+    for c in range(6):
+        assert rel_l1(q[..., c], run_fixtures[name + "/q"][..., c]) <= RUN_L1_TOL
+
+
+def test_fused_uniform_state_unchanged():
+    p = HydroParams(nx=20, ny=18, nz=16, solver=2, limiter=2, boundtype=2, cfl=0.8, atomic=True)
+    d = p.derive()
+    sh = (16, 18, 20)
+    q0 = ics.prim_to_q(d, np.full(sh, 1.3), np.full(sh, 0.2), np.full(sh, -0.1), np.full(sh, 0.05),
+                       np.full(sh, 0.7))
+    with hydro.Hydro(d, mode=hydro.MODE_FUSED) as h:
+        h.upload_q(q0)
+        h.run(3)
+        q = h.download_q()
+    assert np.array_equal(q, q0)
+
+
+def test_fused_matches_staged():
+    p = HydroParams(nx=24, ny=20, nz=18, solver=2, limiter=3, boundtype=3, cfl=0.4, atomic=True)
+    q0 = ics.random_smooth(p, seed=9)
+    out = []
+    for mode in (hydro.MODE_STAGED, hydro.MODE_FUSED):
+        with hydro.Hydro(p, mode=mode) as h:
+            h.upload_q(q0)
+            h.run(4)
+            out.append(h.download_q())
+    for c in range(6):
+        assert rel_linf(out[1][..., c], out[0][..., c]) <= 1e-12
