@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the hydro step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--n 512]
+
+Workload: the 3-D point blast on a 512^3 grid (outflow boundaries,
+gamma 5/3, rho = 1 with 1% seeded noise, p = 1e-5, E = 1 in the cells with
+r < 3.5 cells), HLLC, minmod.  The configured CFL of 0.6 makes the
+reference scheme itself stop on a negative pressure at the blast centre
+(step 29 at 128^3, bit-identical in the CPU oracle and on the GPU), so the
+timed run uses CFL 0.4, which completes; the work per cell-update does not
+depend on the CFL number.  A "step" is one pass of the hydro
+step over the grid (primitive, bound, hydrostep, compevo, sweep), so a step
+is n^3 cell-updates.  The input is resident in HBM before the timed region.
+
+Printed: one JSON line with the headline number (FP64 fused path) and,
+alongside, the roofline of the dominant kernel (HIP-event timing of each
+sweep launch on the library's stream), and the reference's CPU hot path
+timed on the host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {"fp64": 96.0, "fp32_mixed": 48.0}  # read + write 6 conserved values
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_gpu(n, steps, warmup, precision_name="fp64", solver=2, cfl=0.4):
+    from evora_amd import hydro, ics
+
+    p = ics.blast_params(n, solver=solver, cfl=cfl)
+    d = p.derive()
+    log(f"building {n}^3 blast initial condition")
+    q0 = ics.blast(p)
+    prec = hydro.FP64 if precision_name == "fp64" else hydro.FP32_MIXED
+    h = hydro.Hydro(d, precision=prec, mode=hydro.MODE_FUSED)
+    log(f"uploading {q0.nbytes / 1e9:.2f} GB; device bytes {h.device_bytes() / 1e9:.2f} GB")
+    h.upload_q(q0)
+    del q0
+    t = 0.0
+    for _ in range(warmup):
+        t, dt, go = h.step(t, p.tend)
+    h.synchronize()
+    h.set_timing(True)
+    log(f"warmup done ({warmup} steps); timing {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t, dt, go = h.step(t, p.tend)
+    h.synchronize()
+    wall = time.perf_counter() - t0
+    kms, launches = h.kernel_stats()
+    sums = h.sums()
+    h.close()
+    cells = d.ncells
+    res = dict(
+        value=cells * steps / wall,
+        ms_per_step=1e3 * wall / steps,
+        kernel_ms=kms / max(launches, 1),
+        launches=launches,
+        t_end=t,
+        mass=float(sums[0] * d.dx * d.dy * d.dz),
+    )
+    log(f"{precision_name}: {res['value'] / 1e9:.2f} G cell-updates/s, {res['ms_per_step']:.3f} ms/step, "
+        f"kernel {res['kernel_ms']:.3f} ms")
+    return res
+
+
+def cpu_baseline(n, cores, nsteps=2):
+    """The reference's own hot path (oracle/_ref/evref, compiled from the
+    reference Fortran at -O3) on `cores` processes, each advancing one z-slab
+    of the n^3 blast with its own ghost fill (the reference's z-slab split,
+    parallel.f90:53-61, without the halo exchange), `nsteps` steps.  Returns
+    cell-updates/s over the slowest process's step-loop time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from evora_amd import ics
+
+    if not O.have_ref():
+        return None
+    p = ics.blast_params(n)
+    q = ics.blast(p)
+    d = p.derive()
+    bounds = []
+    mz = 0
+    for i in range(cores):  # parallel.f90:55-60
+        lz = mz + 1
+        mz = lz + (n - lz) // (cores - i)
+        bounds.append((lz, mz))
+    procs = []
+    with tempfile.TemporaryDirectory() as td:
+        for r, (lz, mz) in enumerate(bounds):
+            sp = ics.blast_params(n)
+            sp.nz = mz - lz + 1
+            sp.zmin = d.z0 - 0.5 * d.dz + (lz - 1) * d.dz
+            sp.zmax = sp.zmin + sp.nz * d.dz
+            para = os.path.join(td, f"para{r}")
+            with open(para, "w") as fh:
+                fh.write(sp.to_namelist())
+            fin = os.path.join(td, f"in{r}.bin")
+            np.ascontiguousarray(q[lz - 1:mz]).tofile(fin)
+            procs.append((para, fin, os.path.join(td, f"out{r}.bin")))
+        del q
+        log(f"cpu baseline: {cores} reference processes x {nsteps} steps on {n}^3 z-slabs")
+        running = [subprocess.Popen([O.REF_PATH, pa, "run", fi, fo, str(nsteps)], stdout=subprocess.PIPE,
+                                    stderr=subprocess.STDOUT, text=True, cwd=td) for pa, fi, fo in procs]
+        secs = []
+        for pr in running:
+            out, _ = pr.communicate(timeout=900)
+            s = [float(x.split()[1]) for x in out.splitlines() if x.startswith("LOOP_SECONDS")]
+            steps_done = sum(1 for x in out.splitlines() if x.startswith("STEP"))
+            if not s or steps_done != nsteps:
+                log("cpu baseline process failed:\n" + out[-2000:])
+                return None
+            secs.append(s[0])
+    slowest = max(secs)
+    return dict(value=n ** 3 * nsteps / slowest, unit="cell-updates/s", cores=cores, kind="reference",
+                sample=f"{nsteps} steps of the {n}^3 HLLC blast, one reference process per z-slab "
+                       f"({cores} slabs, own ghost fill, no halo exchange), slowest slab {slowest:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--cpu-cores", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the hydro step runs as one rank on one GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    fp64 = run_gpu(args.n, args.steps, args.warmup, "fp64", cfl=args.cfl)
+    achieved = BYTES_PER_CELL["fp64"] * args.n ** 3 / (fp64["kernel_ms"] * 1e-3) / 1e9
+    forms = {"fp64": dict(value=fp64["value"], ms_per_step=fp64["ms_per_step"], kernel_ms=fp64["kernel_ms"],
+                          hbm_frac=achieved / HBM_PEAK_GBS)}
+    if not args.fp64_only:
+        mx = run_gpu(args.n, args.steps, args.warmup, "fp32_mixed", cfl=args.cfl)
+        ach = BYTES_PER_CELL["fp32_mixed"] * args.n ** 3 / (mx["kernel_ms"] * 1e-3) / 1e9
+        forms["fp32_mixed"] = dict(value=mx["value"], ms_per_step=mx["ms_per_step"], kernel_ms=mx["kernel_ms"],
+                                   hbm_frac=ach / HBM_PEAK_GBS, speedup_vs_fp64=mx["value"] / fp64["value"],
+                                   mass_drift=(mx["mass"] - fp64["mass"]) / fp64["mass"])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.n, min(args.cpu_cores, os.cpu_count() or 1), args.cpu_steps)
+    line = {
+        "metric": "cell-updates/s, 512^3 HLLC, FP32-mixed vs FP64 on 1 GPU; % of HBM roofline",
+        "value": fp64["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded point-blast initial condition)",
+        "config": {"workload": f"3-D point blast {args.n}^3, HLLC, minmod, CFL {args.cfl}, outflow, FP64 fused sweep",
+                   "cells": args.n ** 3, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fused (FP64)", "kernel_ms": fp64["kernel_ms"],
+                     "bytes_per_cell_update": BYTES_PER_CELL["fp64"]},
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -29,7 +29,7 @@ COMMON = [
 UNITS = {
     "api.hip": [],
     "staged.hip": ["-ffp-contract=off"],
-    "fused.hip": ["-ffp-contract=off"],
+    "fused.hip": [],
 }
 
 

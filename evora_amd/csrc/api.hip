@@ -221,6 +221,11 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
   k.dx = cfg->dx; k.dy = cfg->dy; k.dz = cfg->dz;
   k.dxinv = cfg->dxinv; k.dyinv = cfg->dyinv; k.dzinv = cfg->dzinv;
   k.mach_fp64 = cfg->mach_fp64;
+  {
+    // Mach M  <=>  kin/e = g (g-1) M^2 / 2  <=>  kin/E = R / (1 + R)
+    const double R = 0.5 * cfg->g * cfg->g1 * cfg->mach_fp64 * cfg->mach_fp64;
+    k.kin_frac64 = cfg->mach_fp64 > 0 ? R / (1.0 + R) : -1.0;
+  }
   h->grid = Grid{cfg->nx, cfg->ny, cfg->nz};
   h->qelem = mixed(h) ? sizeof(float) : sizeof(double);
   h->mode = fused_supported(*cfg) ? EVH_MODE_FUSED : EVH_MODE_STAGED;

@@ -30,6 +30,7 @@ struct evh_ctx {
   evh::Grid grid;
   int device = 0;
   int mode = EVH_MODE_FUSED;
+  int zchunk = 0;  // z planes per fused block (0: default)
   hipStream_t stream = nullptr;
   hipEvent_t events[8] = {};
 

@@ -1,43 +1,101 @@
 // fused.hip -- the whole 3-D hydro step in one HBM pass.
 //
-// One kernel reads q^n once and writes q^{n+1} once.  Each block owns a
-// 14x14 column tile (plus a one-cell halo ring, 16x16 threads = one column
-// each) and marches up a chunk of z planes, keeping the primitive state of
-// planes k-1, k, k+1 of its column in registers and plane k with a two-cell
-// ring in LDS.  Per plane it
-//   1. recovers w = q2w(q) of plane k+1 (primitive, comp.f90:26-85) with the
-//      ghost cells of `bound` folded into the load index (boundary.f90),
-//   2. forms the limited slopes in x, y, z and the half-step predictor evo
-//      (compevo, muscl.f90:45-113) for plane k,
-//   3. solves the x and y faces of plane k and the z face k-1/2
-//      (riemann.f90), and
-//   4. finishes plane k-1: q + dtx*(Fx) then + dty*(Fy) then + dtz*(Fz) in
-//      the reference's x, y, z order (muscl.f90:191-272, integration :287),
-//      and folds primitive + the CFL limit of the new state into a min
-//      reduction for the next step's hydrostep (timestep.f90:26-57).
-// w, evo and s never exist in HBM.
+// One kernel reads q^n once and writes q^{n+1} once; w, evo and s never
+// exist in HBM.  A block owns a 16x12 column tile and marches up a chunk of
+// z planes.  Waves 0-2 own the 192 interior columns one to one; wave 3 (the
+// "ring wave") owns the 56 columns of the first halo ring and loads the
+// second ring and the four corners, so every interior lane does useful work
+// in every stage.  Per plane k:
+//   S1  primitive (comp.f90:26-85) of plane k+1 for every owned column, with
+//       the ghost cells of `bound` (boundary.f90) folded into the load index;
+//       plane k goes to LDS with both halo rings.
+//   S2  limited slopes in x, y, z and the predictor evo (muscl.f90:45-169)
+//       of plane k; the x/y face states go to LDS per face; the interior
+//       lanes solve their z face k-1/2 (riemann.f90) from registers.
+//   S3  x and y faces of plane k: interior lane (i,j) solves its left and
+//       bottom faces, the ring wave the 16 right-edge and 16 top-edge faces.
+//       Each flux overwrites the two states it came from.
+//   S4  plane k-1 is finished as q + dtx*(Fx) + dty*(Fy) + dtz*(Fz), in the
+//       reference's x, y, z order (muscl.f90:191-272, integration :287), and
+//       primitive + the CFL limit of the new state feed a min reduction for
+//       the next hydrostep (timestep.f90:26-57).
+// Template parameters: T compute type, S storage type of q (double/double
+// for the FP64 form, float/float for the FP32-mixed form).
 #include "evh_internal.h"
+#include "physics_fast.h"
 
 namespace evh {
 namespace {
 
-constexpr int BX = 16, BY = 16;       // threads per block (columns incl. halo ring)
-constexpr int TX = BX - 2, TY = BY - 2;  // interior tile
-constexpr int NT = BX * BY;
-constexpr int NRING = 2 * TX + 2 * TY;   // second halo ring (x/y slopes of the first ring)
-constexpr int PW = BX + 2;               // LDS plane pitch (ring at -1 and BX)
+constexpr int TX = 16, TY = 12;        // interior tile
+constexpr int NI = TX * TY;            // interior lanes (waves 0-2)
+constexpr int NT = NI + 64;            // + the ring wave
+constexpr int NR1 = 2 * TX + 2 * TY;   // first-ring columns (56)
+constexpr int EW = TX + 4;             // extended plane pitch (x from -2 to TX+1)
+constexpr int EA = EW * (TY + 4);      // extended plane area
+constexpr int NFX = (TX + 1) * TY;     // x faces of a plane
+constexpr int NFY = TX * (TY + 1);     // y faces of a plane
 
 struct Geo {
   int nx, ny, nz;
   long long n;
   int zchunk;
-  int tiles_x;
+  int tiles_x, tiles_y, nchunks;
 };
 
-// load q at (gx, gy, gz) (ghost-mapped) and recover w; returns the q used (dual-energy fixed)
+__device__ __forceinline__ int eidx(int x, int y) { return (x + 2) + EW * (y + 2); }
+
+// mixed form: FP64 pressure recovery where the flow is fast
+template <typename T>
+struct Prim;
+
+template <>
+struct Prim<double> {
+  __device__ static __forceinline__ int q2w(const KParams &P, double q[6], double w[5], double &rinv) {
+    return fast::q2w(P, q, w, rinv);
+  }
+};
+
+template <>
+struct Prim<float> {
+  // FP32 recovery, except where the subtraction E - kin cancels: there
+  // (kin/E above the Mach threshold) p = g1 (E - kin) is evaluated in FP64.
+  // With dualenergy on, the entropy branch (kin/E > de_thresh1,
+  // comp.f90:61-64) recovers p = S rho**g1 and E = kin + p/g1 in FP64.
+  __device__ static __forceinline__ int q2w(const KParams &P, float q[6], float w[5], float &rinv) {
+    rinv = fm::rcp(q[0]);
+    w[0] = q[0];
+    w[1] = q[1] * rinv;
+    w[2] = q[2] * rinv;
+    w[3] = q[3] * rinv;
+    const float m2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    const float kin = 0.5f * m2 * rinv;
+    const bool fast_flow = kin > float(P.kin_frac64) * q[4];
+    float pr;
+    if (P.dualenergy && kin * fm::rcp(q[4]) > float(P.de_thresh1)) {
+      const double r64 = double(q[0]);
+      const double k64 = 0.5 * (double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]) / r64;
+      const double p64 = double(q[5]) * fm::pow_pos(r64, P.g1);
+      q[4] = float(k64 + p64 * P.g1inv);
+      pr = float(p64);
+    } else {
+      if (fast_flow) {
+        const double k64 = 0.5 * (double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]) / double(q[0]);
+        pr = float(P.g1 * (double(q[4]) - k64));
+      } else {
+        pr = float(P.g1) * (q[4] - kin);
+      }
+      if (P.dualenergy) q[5] = pr * fm::pow_pos(q[0], -float(P.g1));
+    }
+    w[4] = pr;
+    return (pr <= 0.f) ? kErrNegPressure : kOk;
+  }
+};
+
+// load q at (gx, gy, gz) (ghost-mapped) and recover w
 template <typename S, typename T>
 __device__ __forceinline__ void load_cell(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
-                                          int gz, T qc[6], T wc[5]) {
+                                          int gz, T qc[6], T wc[5], T &rinv) {
   const bool per = P.boundtype == 2;
   const int cx = gx < -2 ? -2 : (gx > G.nx + 1 ? G.nx + 1 : gx);
   const int cy = gy < -2 ? -2 : (gy > G.ny + 1 ? G.ny + 1 : gy);
@@ -47,7 +105,7 @@ __device__ __forceinline__ void load_cell(const KParams &P, const Geo &G, const 
   const long long c = (long long)sx + (long long)G.nx * ((long long)sy + (long long)G.ny * sz);
 #pragma unroll
   for (int m = 0; m < 6; m++) qc[m] = T(q[m * G.n + c]);
-  q2w(P, qc, wc);
+  Prim<T>::q2w(P, qc, wc, rinv);
   if (P.boundtype == 3 && cx >= G.nx) {  // reflecting upper x (boundary.f90:57-64)
     wc[1] = -wc[1];
     wc[2] = -wc[2];
@@ -55,158 +113,223 @@ __device__ __forceinline__ void load_cell(const KParams &P, const Geo &G, const 
   }
 }
 
-template <int SOLVER, int LIMITER, typename S>
+template <typename T>
+__device__ __forceinline__ void store_w(T (*W)[EA], int slot, const T w[5]) {
+#pragma unroll
+  for (int m = 0; m < 5; m++) W[m][slot] = w[m];
+}
+
+template <typename T, typename S, int SOLVER, int LIMITER>
 __global__ void __launch_bounds__(NT) k_fused(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout,
                                               double dt, Status *st) {
-  using T = double;
-  __shared__ T wpl[5][PW * PW];   // plane k, with the second ring
-  __shared__ T xs[5][NT];         // right-face x states
-  __shared__ T ys[5][NT];         // top-face y states
-  __shared__ T fx[6][NT];         // flux through each column's left x face
-  __shared__ T fy[6][NT];         // flux through each column's bottom y face
+  __shared__ T W[5][EA];      // w of plane k, both halo rings
+  __shared__ T XB[10][NFX];   // x face (fi, j), fi = 0..TX: [0,5) left state, [5,10) right state; then flux
+  __shared__ T YB[10][NFY];   // y face (i, fj), fj = 0..TY
 
   const int tid = threadIdx.x;
-  const int tx = tid % BX, ty = tid / BX;
-  const int bx = blockIdx.x % G.tiles_x, by = blockIdx.x / G.tiles_x;
-  const int x0 = bx * TX, y0 = by * TY;
-  const int gx = x0 - 1 + tx, gy = y0 - 1 + ty;
-  const int z0 = blockIdx.y * G.zchunk;
+  const bool ring = tid >= NI;
+  // tile of this block
+  const int tile = blockIdx.x % (G.tiles_x * G.tiles_y);
+  const int chunk = blockIdx.x / (G.tiles_x * G.tiles_y);
+  const int x0 = (tile % G.tiles_x) * TX, y0 = (tile / G.tiles_x) * TY;
+  const int z0 = chunk * G.zchunk;
   const int z1 = min(z0 + G.zchunk, G.nz);
-  const bool interior = tx >= 1 && tx <= TX && ty >= 1 && ty <= TY && gx < G.nx && gy < G.ny;
 
-  // ring cell of this thread (tid < NRING)
-  int rgx = 0, rgy = 0, rslot = -1;
-  if (tid < NRING) {
-    int r = tid;
-    if (r < TY) { rgx = x0 - 2; rgy = y0 + r; rslot = (r + 2) * PW + 0; }
-    else if ((r -= TY) < TY) { rgx = x0 + TX + 1; rgy = y0 + r; rslot = (r + 2) * PW + (BX + 1); }
-    else if ((r -= TY) < TX) { rgx = x0 + r; rgy = y0 - 2; rslot = 0 * PW + (r + 2); }
-    else { r -= TX; rgx = x0 + r; rgy = y0 + TY + 1; rslot = (BY + 1) * PW + (r + 2); }
+  // owned column (interior cell, or first-ring cell for the ring wave); the
+  // ring wave also loads one second-ring cell or first-ring corner per lane
+  int cx, cy, side = -1, r2x = 0, r2y = 0;
+  const int lane = tid - NI;
+  if (!ring) {
+    cx = tid % TX;
+    cy = tid / TX;
+  } else if (lane < TY) {
+    side = 0; cx = -1; cy = lane; r2x = -2; r2y = lane;
+  } else if (lane < 2 * TY) {
+    side = 1; cx = TX; cy = lane - TY; r2x = TX + 1; r2y = cy;
+  } else if (lane < 2 * TY + TX) {
+    side = 2; cx = lane - 2 * TY; cy = -1; r2x = cx; r2y = -2;
+  } else if (lane < NR1) {
+    side = 3; cx = lane - 2 * TY - TX; cy = TY; r2x = cx; r2y = TY + 1;
+  } else {  // lanes NR1..NR1+3 load the first-ring corners, the rest idle
+    const int c = lane - NR1;
+    cx = (c & 1) ? TX : -1;
+    cy = (c & 2) ? TY : -1;
+    r2x = cx;
+    r2y = cy;
+    if (c >= 4) r2x = -3;  // no second-ring job
   }
-  const int mslot = (ty + 1) * PW + (tx + 1);
+  const bool owns_col = !ring || side >= 0;
+  const int gx = x0 + cx, gy = y0 + cy;
+  const bool inside = !ring && gx < G.nx && gy < G.ny;
+  const int me = eidx(cx, cy);
 
-  const T dt2 = -0.5 * dt;
-  const T dtx = dt * P.dxinv, dty = dt * P.dyinv, dtz = dt * P.dzinv;
+  const T dt2 = T(-0.5 * dt);
+  const T dtx = T(dt * P.dxinv), dty = T(dt * P.dyinv), dtz = T(dt * P.dzinv);
+  const T half = T(0.5);
 
-  T wm[5], wc[5], wp[5], qn[6], qc[6], tmpq[6];
+  // FP32 keeps the base q of the update in registers; FP64 reloads it from
+  // cache in S4 (it was read one plane earlier), which saves 24 VGPRs
+  constexpr bool kKeepQ = sizeof(T) == 4;
+  T wm[5], wc[5], wp[5], qn[6], qc[6], tq[6];
+  T rm, rc, rp;
   T part[6], zr_prev[5], fz_prev[6];
   double vmin = 1e6;
 
-  load_cell<S, T>(P, G, qin, gx, gy, z0 - 2, tmpq, wm);
-  load_cell<S, T>(P, G, qin, gx, gy, z0 - 1, tmpq, wc);
+  load_cell<S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
+  load_cell<S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
 #pragma unroll
-  for (int m = 0; m < 6; m++) { part[m] = 0; fz_prev[m] = 0; qc[m] = 0; }
+  for (int m = 0; m < 6; m++) { part[m] = T(0); fz_prev[m] = T(0); qc[m] = T(0); qn[m] = T(0); }
 #pragma unroll
-  for (int m = 0; m < 5; m++) zr_prev[m] = 0;
+  for (int m = 0; m < 5; m++) zr_prev[m] = T(0);
 
   for (int k = z0 - 1; k <= z1; k++) {
-    // 1. primitive of plane k+1 for this column
-    load_cell<S, T>(P, G, qin, gx, gy, k + 1, qn, wp);
-    // 2. ring cells of plane k
-    if (rslot >= 0) {
-      T rq[6], rw[5];
-      load_cell<S, T>(P, G, qin, rgx, rgy, k, rq, rw);
-#pragma unroll
-      for (int m = 0; m < 5; m++) wpl[m][rslot] = rw[m];
+    const bool out_plane = k >= z0 && k < z1;
+    // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
+    if (owns_col) {
+      if (kKeepQ) load_cell<S, T>(P, G, qin, gx, gy, k + 1, qn, wp, rp);
+      else load_cell<S, T>(P, G, qin, gx, gy, k + 1, tq, wp, rp);
     }
-#pragma unroll
-    for (int m = 0; m < 5; m++) wpl[m][mslot] = wc[m];
+    if (ring && r2x > -3) {
+      T rw[5], rr;
+      load_cell<S, T>(P, G, qin, x0 + r2x, y0 + r2y, k, tq, rw, rr);
+      store_w(W, eidx(r2x, r2y), rw);
+    }
+    if (owns_col) store_w(W, me, wc);
     __syncthreads();
 
-    // 3. slopes and predictor of plane k (muscl.f90:93-107)
-    T sx[5], sy[5], sz[5];
+    // ---- S2: slopes, predictor and face states of plane k
+    T sx[5], sy[5], sz[5], ev[5];
+    T fz[6];
+    if (owns_col) {
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-      sx[m] = slope3<LIMITER>(wpl[m][mslot - 1], wc[m], wpl[m][mslot + 1]);
-      sy[m] = slope3<LIMITER>(wpl[m][mslot - PW], wc[m], wpl[m][mslot + PW]);
-      sz[m] = slope3<LIMITER>(wm[m], wc[m], wp[m]);
+      sx[m] = fast::slope3<LIMITER>(W[m][me - 1], wc[m], W[m][me + 1]);
+      sy[m] = fast::slope3<LIMITER>(W[m][me - EW], wc[m], W[m][me + EW]);
+      sz[m] = fast::slope3<LIMITER>(wm[m], wc[m], wp[m]);
     }
-    T ev[5];
     if (P.secondorder) {
       T a1[5], a2[5], a3[5];
-      predictor_dir<0>(P, wc, sx, a1);
-      predictor_dir<1>(P, wc, sy, a2);
-      predictor_dir<2>(P, wc, sz, a3);
+      fast::predictor_dir<0>(P, wc, rc, sx, a1);
+      fast::predictor_dir<1>(P, wc, rc, sy, a2);
+      fast::predictor_dir<2>(P, wc, rc, sz, a3);
+      const T ix = T(P.dxinv), iy = T(P.dyinv), iz = T(P.dzinv);
 #pragma unroll
-      for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * P.dxinv + a2[m] * P.dyinv + a3[m] * P.dzinv);
+      for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * ix + a2[m] * iy + a3[m] * iz);
     } else {
 #pragma unroll
-      for (int m = 0; m < 5; m++) { ev[m] = 0; sx[m] = 0; sy[m] = 0; sz[m] = 0; }
+      for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }
     }
-    // face states (muscl.f90:207-210): right/top = w + 0.5 s + evo, left/bottom = w - 0.5 s + evo
-    T xl[5], yl[5], zl[5], zr[5];
+    // face states (muscl.f90:207-210): w + 0.5 s + evo on the high side, w - 0.5 s + evo on the low side
+    if (!ring) {
+      const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
 #pragma unroll
-    for (int m = 0; m < 5; m++) {
-      xs[m][tid] = wc[m] + 0.5 * sx[m] + ev[m];
-      ys[m][tid] = wc[m] + 0.5 * sy[m] + ev[m];
-      xl[m] = wc[m] - 0.5 * sx[m] + ev[m];
-      yl[m] = wc[m] - 0.5 * sy[m] + ev[m];
-      zl[m] = wc[m] - 0.5 * sz[m] + ev[m];
-      zr[m] = wc[m] + 0.5 * sz[m] + ev[m];
-    }
-    __syncthreads();
-
-    const bool out_plane = k >= z0 && k < z1;
-    // 4. x and y faces of plane k
-    if (out_plane) {
-      if (tx >= 1 && ty >= 1 && ty <= TY) {
-        T wl[5], f[6];
-#pragma unroll
-        for (int m = 0; m < 5; m++) wl[m] = xs[m][tid - 1];
-        const int e = riemann<SOLVER, 1>(P, wl, xl, f);
-        if (e && gx <= G.nx && gy < G.ny) report(st, e, (unsigned long long)(gx + (long long)G.nx * (gy + (long long)G.ny * k)));
-#pragma unroll
-        for (int m = 0; m < 6; m++) fx[m][tid] = f[m];
+      for (int m = 0; m < 5; m++) {
+        XB[5 + m][fxl] = wc[m] - half * sx[m] + ev[m];       // right state of the left face
+        XB[m][fxl + 1] = wc[m] + half * sx[m] + ev[m];       // left state of the right face
+        YB[5 + m][fyb] = wc[m] - half * sy[m] + ev[m];
+        YB[m][fyb + TX] = wc[m] + half * sy[m] + ev[m];
       }
-      if (ty >= 1 && tx >= 1 && tx <= TX) {
-        T wl[5], f[6];
+    } else if (side == 0) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) wl[m] = ys[m][tid - BX];
-        const int e = riemann<SOLVER, 2>(P, wl, yl, f);
-        if (e && gx < G.nx && gy <= G.ny) report(st, e, (unsigned long long)(gx + (long long)G.nx * (gy + (long long)G.ny * k)));
+      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = wc[m] + half * sx[m] + ev[m];
+    } else if (side == 1) {
 #pragma unroll
-        for (int m = 0; m < 6; m++) fy[m][tid] = f[m];
-      }
+      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = wc[m] - half * sx[m] + ev[m];
+    } else if (side == 2) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) YB[m][cx] = wc[m] + half * sy[m] + ev[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = wc[m] - half * sy[m] + ev[m];
     }
-    __syncthreads();
-
-    if (interior) {
-      // 5. z face k-1/2, finish plane k-1
+    // z face k-1/2 (interior lanes, from registers)
+    if (!ring) {
+      T zl[5], zr[5];
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        zl[m] = wc[m] - half * sz[m] + ev[m];
+        zr[m] = wc[m] + half * sz[m] + ev[m];
+      }
       if (k >= z0) {
-        T f[6];
-        const int e = riemann<SOLVER, 3>(P, zr_prev, zl, f);
-        if (e) report(st, e, (unsigned long long)(gx + (long long)G.nx * (gy + (long long)G.ny * k)));
-        if (k - 1 >= z0) {
-          T o[6];
-#pragma unroll
-          for (int m = 0; m < 6; m++) o[m] = part[m] + dtz * (fz_prev[m] - f[m]);
-          const long long c = (long long)gx + (long long)G.nx * ((long long)gy + (long long)G.ny * (k - 1));
-#pragma unroll
-          for (int m = 0; m < 6; m++) qout[m * G.n + c] = S(o[m]);
-          // primitive + CFL of the new state (next step's primitive/hydrostep)
-          T onew[6], wn[5];
-#pragma unroll
-          for (int m = 0; m < 6; m++) onew[m] = T(S(o[m]));
-          const int ep = q2w(P, onew, wn);
-          if (ep) report_next(st, ep, (unsigned long long)c);
-          vmin = dmin(vmin, cfl_cell(P, wn));
-        }
-#pragma unroll
-        for (int m = 0; m < 6; m++) fz_prev[m] = f[m];
-      }
-      // 6. x then y differences of plane k
-      if (out_plane) {
-#pragma unroll
-        for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (fx[m][tid] - fx[m][tid + 1]);
-#pragma unroll
-        for (int m = 0; m < 6; m++) part[m] = part[m] + dty * (fy[m][tid] - fy[m][tid + BX]);
+        const int e = fast::riemann<SOLVER, 3>(P, zr_prev, zl, fz);
+        if (e && inside) report(st, e, (unsigned long long)(gx + (long long)G.nx * (gy + (long long)G.ny * k)));
       }
 #pragma unroll
       for (int m = 0; m < 5; m++) zr_prev[m] = zr[m];
     }
-    // 7. rotate the column window
+    }  // owns_col
+    __syncthreads();
+
+    // ---- S3: x and y faces of plane k
+    if (out_plane) {
+      int fxi = -1, fyi = -1;
+      if (!ring) {
+        fxi = cx + (TX + 1) * cy;
+        fyi = cx + TX * cy;
+      } else {
+        if (lane < TY) fxi = TX + (TX + 1) * lane;          // right-edge x faces
+        else if (lane < TY + TX) fyi = (lane - TY) + TX * TY;  // top-edge y faces
+      }
+      if (fxi >= 0) {
+        T wl[5], wr[5], f[6];
+#pragma unroll
+        for (int m = 0; m < 5; m++) { wl[m] = XB[m][fxi]; wr[m] = XB[5 + m][fxi]; }
+        const int e = fast::riemann<SOLVER, 1>(P, wl, wr, f);
+        const int fgx = x0 + fxi % (TX + 1), fgy = y0 + fxi / (TX + 1);
+        if (e && fgx <= G.nx && fgy < G.ny) report(st, e, (unsigned long long)(min(fgx, G.nx - 1) + (long long)G.nx * (fgy + (long long)G.ny * k)));
+#pragma unroll
+        for (int m = 0; m < 6; m++) XB[m][fxi] = f[m];
+      }
+      if (fyi >= 0) {
+        T wl[5], wr[5], f[6];
+#pragma unroll
+        for (int m = 0; m < 5; m++) { wl[m] = YB[m][fyi]; wr[m] = YB[5 + m][fyi]; }
+        const int e = fast::riemann<SOLVER, 2>(P, wl, wr, f);
+        const int fgx = x0 + fyi % TX, fgy = y0 + fyi / TX;
+        if (e && fgx < G.nx && fgy <= G.ny) report(st, e, (unsigned long long)(fgx + (long long)G.nx * (min(fgy, G.ny - 1) + (long long)G.ny * k)));
+#pragma unroll
+        for (int m = 0; m < 6; m++) YB[m][fyi] = f[m];
+      }
+    }
+    __syncthreads();
+
+    // ---- S4: finish plane k-1, x/y differences of plane k
+    if (!ring) {
+      if (k >= z0) {
+        if (k - 1 >= z0 && inside) {
+          T o[6];
+#pragma unroll
+          for (int m = 0; m < 6; m++) o[m] = part[m] + dtz * (fz_prev[m] - fz[m]);
+          const long long c = (long long)gx + (long long)G.nx * ((long long)gy + (long long)G.ny * (k - 1));
+#pragma unroll
+          for (int m = 0; m < 6; m++) qout[m * G.n + c] = S(o[m]);
+          T on[6], wn[5], rn;
+#pragma unroll
+          for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
+          const int ep = Prim<T>::q2w(P, on, wn, rn);
+          if (ep) report_next(st, ep, (unsigned long long)c);
+          vmin = dmin(vmin, fast::cfl_cell(P, wn, rn));
+        }
+#pragma unroll
+        for (int m = 0; m < 6; m++) fz_prev[m] = fz[m];
+      }
+      if (out_plane) {
+        const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
+        if (!kKeepQ) {  // base q(k) of this column, dual-energy corrected as in S1
+          T wt[5], rt;
+          load_cell<S, T>(P, G, qin, gx, gy, k, qc, wt, rt);
+        }
+#pragma unroll
+        for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
+#pragma unroll
+        for (int m = 0; m < 6; m++) part[m] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+      }
+    }
+    // rotate the column window
 #pragma unroll
     for (int m = 0; m < 5; m++) { wm[m] = wc[m]; wc[m] = wp[m]; }
+    rm = rc;
+    rc = rp;
 #pragma unroll
     for (int m = 0; m < 6; m++) qc[m] = qn[m];
   }
@@ -215,7 +338,8 @@ __global__ void __launch_bounds__(NT) k_fused(KParams P, Geo G, const S *__restr
   if ((tid & 63) == 0) atomic_min_pos(&st->vmin_bits, vmin);
 }
 
-// primitive + CFL min over q without materialising w (first step, after uploads)
+// primitive + CFL min over q without materialising w (first step, after
+// uploads): exact IEEE arithmetic, so the first dt equals the reference's.
 template <typename S>
 __global__ void __launch_bounds__(256) k_cfl(KParams P, Geo G, const S *__restrict__ q, Status *st) {
   double vmin = 1e6;
@@ -237,47 +361,55 @@ Geo make_geo(const evh_ctx *h) {
   g.ny = h->grid.ny;
   g.nz = h->grid.nz;
   g.n = h->grid.n();
-  g.zchunk = 64;
+  g.zchunk = h->zchunk > 0 ? h->zchunk : 128;
   if (g.zchunk > g.nz) g.zchunk = g.nz;
   g.tiles_x = (g.nx + TX - 1) / TX;
+  g.tiles_y = (g.ny + TY - 1) / TY;
+  g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
   return g;
 }
 
-template <int SOLVER, int LIMITER>
+template <typename T, typename S, int SOLVER, int LIMITER>
 void launch_fused_t(evh_ctx *h, double dt) {
   const Geo g = make_geo(h);
-  const int tiles_y = (g.ny + TY - 1) / TY;
-  const dim3 grid(g.tiles_x * tiles_y, (g.nz + g.zchunk - 1) / g.zchunk);
-  const double *qin = (const double *)h->qbuf[h->cur];
-  double *qout = (double *)h->qbuf[1 - h->cur];
-  hipLaunchKernelGGL((k_fused<SOLVER, LIMITER, double>), grid, dim3(NT), 0, h->stream, h->kp, g, qin, qout, dt, h->st);
+  const int blocks = g.tiles_x * g.tiles_y * g.nchunks;
+  hipLaunchKernelGGL((k_fused<T, S, SOLVER, LIMITER>), dim3(blocks), dim3(NT), 0, h->stream, h->kp, g,
+                     (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st);
+}
+
+template <typename T, typename S>
+hipError_t launch_fused_prec(evh_ctx *h, double dt) {
+  const int s = h->kp.solver, l = h->kp.limiter;
+  if (s == 1 && l == 1) launch_fused_t<T, S, 1, 1>(h, dt);
+  else if (s == 1 && l == 2) launch_fused_t<T, S, 1, 2>(h, dt);
+  else if (s == 1 && l == 3) launch_fused_t<T, S, 1, 3>(h, dt);
+  else if (s == 2 && l == 1) launch_fused_t<T, S, 2, 1>(h, dt);
+  else if (s == 2 && l == 2) launch_fused_t<T, S, 2, 2>(h, dt);
+  else if (s == 2 && l == 3) launch_fused_t<T, S, 2, 3>(h, dt);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 }  // namespace
 
-bool fused_supported(const evh_config &c) {
-  return c.nx >= 2 && c.ny >= 2 && c.nz >= 2 && c.precision == EVH_FP64;
-}
+bool fused_supported(const evh_config &c) { return c.nx >= 2 && c.ny >= 2 && c.nz >= 2; }
 
 hipError_t launch_cfl(evh_ctx *h) {
   const Geo g = make_geo(h);
   long long b = (g.n + 255) / 256;
   if (b > 8192) b = 8192;
-  hipLaunchKernelGGL(k_cfl<double>, dim3((int)b), dim3(256), 0, h->stream, h->kp, g, (const double *)h->qbuf[h->cur],
-                     h->st);
+  if (h->cfg.precision == EVH_FP32_MIXED)
+    hipLaunchKernelGGL(k_cfl<float>, dim3((int)b), dim3(256), 0, h->stream, h->kp, g, (const float *)h->qbuf[h->cur],
+                       h->st);
+  else
+    hipLaunchKernelGGL(k_cfl<double>, dim3((int)b), dim3(256), 0, h->stream, h->kp, g,
+                       (const double *)h->qbuf[h->cur], h->st);
   return hipGetLastError();
 }
 
 hipError_t launch_fused_step(evh_ctx *h, double dt) {
-  const int s = h->kp.solver, l = h->kp.limiter;
-  if (s == 1 && l == 1) launch_fused_t<1, 1>(h, dt);
-  else if (s == 1 && l == 2) launch_fused_t<1, 2>(h, dt);
-  else if (s == 1 && l == 3) launch_fused_t<1, 3>(h, dt);
-  else if (s == 2 && l == 1) launch_fused_t<2, 1>(h, dt);
-  else if (s == 2 && l == 2) launch_fused_t<2, 2>(h, dt);
-  else if (s == 2 && l == 3) launch_fused_t<2, 3>(h, dt);
-  else return hipErrorInvalidValue;
-  return hipGetLastError();
+  if (h->cfg.precision == EVH_FP32_MIXED) return launch_fused_prec<float, float>(h, dt);
+  return launch_fused_prec<double, double>(h, dt);
 }
 
 }  // namespace evh

@@ -34,6 +34,7 @@ struct KParams {
   double g, g1, g1inv, gg1, cfl, de_thresh1;
   double dx, dy, dz, dxinv, dyinv, dzinv;
   double mach_fp64;
+  double kin_frac64;  // FP32-mixed: kin/E above which p is recovered in FP64
 };
 
 // Device status block: first error (code, linear cell) and min reductions.

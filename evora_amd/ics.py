@@ -164,8 +164,10 @@ def ensemble_sizes(nbox=48, seed=7, edges=(64, 96, 128, 160, 192)):
 
 # ----------------------------------------------------------------- supersonic random field
 def cosmo_params(n=256, **kw) -> HydroParams:
+    """Periodic supersonic field, HLLC, CFL 0.5, with the dual-energy switch
+    (comp.f90:60-69) on where E_kin/E > 0.99 (de_thresh = 0.01)."""
     base = dict(nx=n, ny=n, nz=n, boundtype=2, solver=2, limiter=1, cfl=0.5, atomic=True,
-                tstart=0.0, tend=1e30)
+                tstart=0.0, tend=1e30, dualenergy=True, de_thresh=0.01)
     base.update(kw)
     return HydroParams(**base)
 

@@ -17,7 +17,8 @@
 !   mode stage  : in = q; one pass; out = w, evo, s, q_after_sweep, dt
 !   mode riemann: in = n records of (wl(5), wr(5), dir); out = f(6) per record
 !   mode slope  : in = n records of (wl(5), w(5), wr(5)); out = s(5) per record
-! Each step prints "STEP k t dt" to stdout.  Reference errors print their own
+! Each step prints "STEP k t dt" to stdout; the step loop's wall time is
+! printed as "LOOP_SECONDS s" (system_clock, I/O excluded).  Reference errors print their own
 ! "ERROR ..." lines and `stop` (exit status 0), exactly as in the reference.
 !-----------------------------------------------------------------------------
 program evref
@@ -29,6 +30,7 @@ program evref
   real(8), allocatable :: rec(:)
   real(8) :: wl(nw), wr(nw), wc(nw), f(nf), sl(nw)
   integer :: nsteps, k, n, i, u, fsize, dir, k4
+  integer(8) :: c0, c1, crate
   logical :: go, dump
 
   call get_command_argument(2, mode)
@@ -102,6 +104,7 @@ program evref
   dump = .false.
   k4 = 1
   if (trim(mode) .eq. 'stage') nsteps = 1
+  call system_clock(c0, crate)
   do k = 1, nsteps
      if (.not. go) exit
      call primitive(w, q)
@@ -119,6 +122,8 @@ program evref
      call sweep(w, q, evo, s)
      write (*, '(A,I6,2ES25.16)') 'STEP ', k, t, dt
   enddo
+  call system_clock(c1)
+  write (*, '(A,ES16.8)') 'LOOP_SECONDS ', dble(c1 - c0) / dble(crate)
 
   if (trim(mode) .eq. 'stage') then
      write (u) q

@@ -37,6 +37,12 @@ def stage_cases():
                                     cfl=0.4, atomic=True, tend=10.0)
                     seed = 100 * solver + 10 * limiter + bt
                     yield f"stage_{dim}_s{solver}_l{limiter}_b{bt}", p, seed
+    # dual energy (comp.f90:60-69): de_thresh 0.98 puts cells in both branches
+    for solver in (1, 2):
+        for limiter in (1, 2, 3):
+            p = HydroParams(nx=8, ny=7, nz=6, solver=solver, limiter=limiter, boundtype=2, cfl=0.4, atomic=True,
+                            tend=10.0, dualenergy=True, de_thresh=0.98)
+            yield f"stage_3d_s{solver}_l{limiter}_b2_de", p, 500 + 10 * solver + limiter
 
 
 def main():
@@ -76,14 +82,30 @@ def main():
         runs[name + "/q"] = qf
         runs[name + "/t_dt"] = np.array(steps)
         index[name] = dict(kind="run", params=p.__dict__, nsteps=10)
-    # blast with the reference HLL at CFL 0.6: the reference stops on negative pressure
-    p = ics.blast_params(32, solver=1)
-    q0 = ics.blast(p)
-    qf, steps, log = O.ref_run(p, q0, 20)
-    runs["blast32_hll/q0"] = q0
-    runs["blast32_hll/t_dt"] = np.array(steps)
-    index["blast32_hll"] = dict(kind="abort", params=p.__dict__, steps_done=len(steps),
-                                error=[ln.strip() for ln in log.splitlines() if "ERROR" in ln][:1])
+    # cosmo-like field with the dual-energy switch, 10 steps
+    p = ics.cosmo_params(24)
+    q0 = ics.cosmo_field(p)
+    qf, steps, log = O.ref_run(p, q0, 10)
+    assert qf is not None and len(steps) == 10, log[-400:]
+    runs["cosmo24_de/q0"] = q0
+    runs["cosmo24_de/q"] = qf
+    runs["cosmo24_de/t_dt"] = np.array(steps)
+    index["cosmo24_de"] = dict(kind="run", params=p.__dict__, nsteps=10)
+    # configured CFL numbers at which the reference scheme itself stops on a
+    # negative pressure (3-D unsplit, CFL limit taken per direction)
+    aborts = {
+        "blast32_hll": (ics.blast_params(32, solver=1), ics.blast, 20),
+        "kh24_cfl08": (ics.kh_params(24), ics.kelvin_helmholtz, 40),
+        "cosmo16_nodual": (ics.cosmo_params(16, dualenergy=False, de_thresh=0.0), ics.cosmo_field, 20),
+    }
+    for name, (p, gen, nmax) in aborts.items():
+        q0 = gen(p)
+        qf, steps, log = O.ref_run(p, q0, nmax)
+        err = [ln.strip() for ln in log.splitlines() if "ERROR" in ln][:1]
+        assert err, f"{name}: the reference did not stop"
+        runs[name + "/q0"] = q0
+        runs[name + "/t_dt"] = np.array(steps)
+        index[name] = dict(kind="abort", params=p.__dict__, steps_done=len(steps), error=err)
     np.savez_compressed(os.path.join(HERE, "run_fixtures.npz"), **runs)
     with open(os.path.join(HERE, "index.json"), "w") as fh:
         json.dump(index, fh, indent=1, sort_keys=True)

@@ -62,7 +62,36 @@ def test_fused_tile_edges(shape):
         assert rel_linf(q1[..., c], q1_o[..., c]) <= ONE_STEP_TOL, c
 
 
-@pytest.mark.parametrize("name", ["run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3"])
+@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("limiter", [1, 2, 3])
+def test_fused_one_step_dual_energy(solver, limiter):
+    """dualenergy on with de_thresh 0.98: cells take both branches of
+    comp.f90:60-69, and primitive rewrites q(5) or q(6)."""
+    p = HydroParams(nx=21, ny=18, nz=13, solver=solver, limiter=limiter, boundtype=2, cfl=0.4, atomic=True,
+                    dualenergy=True, de_thresh=0.98)
+    q0 = ics.random_smooth(p, seed=300 + 10 * solver + limiter)
+    q1, q1_o, dt, dt_o, _ = one_step(p, q0)
+    assert dt == dt_o
+    for c in range(6):
+        assert rel_linf(q1[..., c], q1_o[..., c]) <= ONE_STEP_TOL, c
+
+
+@pytest.mark.parametrize("name", ["blast32_hll", "kh24_cfl08", "cosmo16_nodual"])
+def test_fused_stops_where_reference_stops(name, run_fixtures, golden_index):
+    """At the configured CFL the reference stops on a negative pressure; the
+    fused path completes the same steps and stops in the next one."""
+    e = golden_index[name]
+    p = HydroParams(**e["params"])
+    with hydro.Hydro(p, mode=hydro.MODE_FUSED) as h:
+        h.upload_q(run_fixtures[name + "/q0"])
+        t, dt, n = h.run(e["steps_done"])
+        assert n == e["steps_done"]
+        with pytest.raises(hydro.EvoraError) as ei:
+            h.run(1, t)
+    assert ei.value.code == 1
+
+
+@pytest.mark.parametrize("name", ["run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3", "cosmo24_de"])
 def test_fused_run_matches_reference_fixture(name, run_fixtures, golden_index):
     p = HydroParams(**golden_index[name]["params"])
     q0 = run_fixtures[name + "/q0"]

@@ -29,7 +29,11 @@ def test_stage_bitwise(name, stage_fixtures, golden_index):
     assert dt == stage_fixtures[name + "/dt"][0]
 
 
-@pytest.mark.parametrize("name", ["sod1000", "run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3"])
+RUNS = ["sod1000", "run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3", "cosmo24_de"]
+ABORTS = ["blast32_hll", "kh24_cfl08", "cosmo16_nodual"]
+
+
+@pytest.mark.parametrize("name", RUNS)
 def test_run_bitwise(name, run_fixtures, golden_index):
     p = params_from_index(golden_index[name])
     q = run_fixtures[name + "/q0"].copy()
@@ -40,17 +44,19 @@ def test_run_bitwise(name, run_fixtures, golden_index):
     assert np.array_equal(q, run_fixtures[name + "/q"])
 
 
-def test_hll_blast_aborts_like_reference(run_fixtures, golden_index):
-    """The reference HLL stops on negative pressure in the CFL-0.6 blast."""
-    e = golden_index["blast32_hll"]
+@pytest.mark.parametrize("name", ABORTS)
+def test_aborts_like_reference(name, run_fixtures, golden_index):
+    """Configured CFL numbers at which the reference stops on a negative
+    pressure: the oracle completes the same steps and stops in the next."""
+    e = golden_index[name]
     p = params_from_index(e)
-    q = run_fixtures["blast32_hll/q0"].copy()
     orc = O.Oracle(p.derive())
-    with pytest.raises(O.OracleError) as ei:
-        orc.run(q, 20)
-    assert ei.value.code == 1  # comp.f90:74-83
-    assert "negative pressure" in e["error"][0]
-    # the same number of completed steps before the stop
-    q = run_fixtures["blast32_hll/q0"].copy()
+    q = run_fixtures[name + "/q0"].copy()
     t, dt, n = orc.run(q, e["steps_done"])
     assert n == e["steps_done"]
+    if n:
+        assert (t, dt) == tuple(run_fixtures[name + "/t_dt"][-1])
+    with pytest.raises(O.OracleError) as ei:
+        orc.run(q, 1, t)
+    assert ei.value.code == 1  # comp.f90:74-83
+    assert "negative pressure" in e["error"][0]
