@@ -1,0 +1,238 @@
+!-----------------------------------------------------------------------------
+! hydro_dispatch.f90 -- runtime-selectable hydro backend for the unchanged
+! driver (source/main.f90:109-203).
+!
+! Defines primitive, bound, hydrostep, compevo and sweep with the reference's
+! implicit interfaces (muscl.f90:26,45,171; boundary.f90:26; timestep.f90:26)
+! and arguments sized from module global, exactly as the reference routines
+! are.  The reference objects are linked with these five symbols renamed to
+! evcpu_* (objcopy --redefine-sym), so that
+!
+!   EVORA_BACKEND=cpu (default)  calls the reference routines unchanged;
+!   EVORA_BACKEND=hip            runs the step on the GPU through libevora_hip.
+!
+! Further environment switches:
+!   EVORA_PRECISION=fp64|mixed   FP64 (default) or the FP32-mixed form
+!   EVORA_MAXSTEPS=N             after N steps, set tend = t in hydrostep so
+!                                that updatetime (timestep.f90:163-166) ends
+!                                the run with a dt = 0 step (both backends)
+!
+! On the GPU the conserved state stays on the device.  It is uploaded at the
+! first primitive call and downloaded after sweep only when the driver is
+! about to read q: when evolution or snapshot will run (the tsnap tests of
+! main.f90:181-195, mirrored with shadow counters) and on the last step
+! (tend - t <= dt, the test of timestep.f90:163).  w, evo and s are never
+! filled on the host; pure hydro never reads them there.  Physics that
+! update q on the host between steps (gravity, chemistry, floors,
+! conduction, supercomoving expansion) and multi-rank runs are refused on
+! the GPU backend.
+!-----------------------------------------------------------------------------
+module hydro_dispatch_state
+  use iso_c_binding
+  use evora_hip
+  implicit none
+
+  integer :: backend = -1            ! -1 undecided, 0 cpu, 1 hip
+  type(c_ptr) :: ctx = c_null_ptr
+  logical :: on_device = .false.
+  logical :: last_step = .false.
+  integer :: maxsteps = -1, nsteps_seen = 0
+  integer :: k2s = 1, k3s = 1        ! shadows of main's evolution/snapshot counters
+  integer(c_int) :: precision = EVH_FP64
+
+contains
+
+  subroutine decide_backend()
+    character(len=64) :: v
+    integer :: l, st
+    if (backend .ge. 0) return
+    backend = 0
+    call get_environment_variable('EVORA_BACKEND', v, l, st)
+    if (st .eq. 0 .and. l .gt. 0) then
+       if (v(1:l) .eq. 'hip' .or. v(1:l) .eq. 'HIP') backend = 1
+    endif
+    call get_environment_variable('EVORA_PRECISION', v, l, st)
+    if (st .eq. 0 .and. l .gt. 0) then
+       if (v(1:l) .eq. 'mixed') precision = EVH_FP32_MIXED
+    endif
+    call get_environment_variable('EVORA_MAXSTEPS', v, l, st)
+    if (st .eq. 0 .and. l .gt. 0) read (v(1:l), *) maxsteps
+  end subroutine decide_backend
+
+  ! reference messages and stop kinds for each status (comp.f90:74-83,
+  ! riemann.f90:37-41,61-66,87-90,111-116,148-151, timestep.f90:51-55,
+  ! muscl.f90:164-167)
+  subroutine hip_check(code)
+    use global
+    integer(c_int), intent(in) :: code
+    integer(c_int) :: c, cell(3)
+    character(kind=c_char) :: msg(256)
+    character(len=256) :: text
+    integer :: i
+    if (code .eq. EVH_OK) return
+    c = evh_last_error(ctx, c, cell, msg, 256_c_int)
+    text = ' '
+    do i = 1, 256
+       if (msg(i) .eq. c_null_char) exit
+       text(i:i) = msg(i)
+    enddo
+    select case (code)
+    case (EVH_ERR_NEG_PRESSURE)
+       print *, 'ERROR negative pressure occured while computing'
+       print *, 'primitive variables!'
+       print *, 'time:', t
+       print *, 'cell:', cell
+       stop
+    case (EVH_ERR_RIEMANN_PRESSURE)
+       if (solver .eq. 1) then
+          print *, 'ERROR: hll solver unphysical pressure'
+       else
+          print *, 'ERROR: hllc solver unphysical pressure'
+       endif
+       print *, 'cell:', cell
+       stop
+    case (EVH_ERR_HLL)
+       print *, 'ERROR: hll solver'
+       stop
+    case (EVH_ERR_HLLC)
+       print *, 'ERROR: hllc solver', cell
+       stop
+    case (EVH_ERR_DT)
+       print *, 'ERROR: hydrostep is negative'
+       print *, 'infinite pressure ?'
+       call cleanstop
+    case (EVH_ERR_SOLVER)
+       print *, 'ERROR choose proper solver!'
+       call cleanstop
+    case (EVH_ERR_LIMITER)
+       print *, 'ERROR: wrong slope limiter!'
+       stop
+    case default
+       print *, 'ERROR: evora_hip: ', trim(text)
+       call cleanstop
+    end select
+  end subroutine hip_check
+
+  subroutine hip_create()
+    use global
+    type(evh_config) :: cfg
+    if (gravity .or. chemistry .or. conduction .or. supercomoving .or. tempfloor .or. jeansfloor &
+         .or. cooldt .or. noncie .or. parallel .or. .not. hydro) then
+       print *, 'ERROR: EVORA_BACKEND=hip runs pure hydro on one rank only'
+       call cleanstop
+    endif
+    cfg%nx = nx ; cfg%ny = ny ; cfg%nz = nz
+    cfg%boundtype = boundtype ; cfg%solver = solver ; cfg%limiter = limiter
+    cfg%secondorder = merge(1, 0, secondorder)
+    cfg%dualenergy = merge(1, 0, dualenergy)
+    cfg%onedimension = merge(1, 0, onedimension)
+    cfg%twodimension = merge(1, 0, twodimension)
+    cfg%precision = precision
+    cfg%g = g ; cfg%g1 = g1 ; cfg%g1inv = g1inv ; cfg%gg1 = gg1 ; cfg%cfl = cfl
+    cfg%de_thresh1 = de_thresh1
+    cfg%dx = dx ; cfg%dy = dy ; cfg%dz = dz
+    cfg%dxinv = dxinv ; cfg%dyinv = dyinv ; cfg%dzinv = dzinv
+    cfg%volume = volume
+    cfg%mach_fp64 = 5.0d0
+    call hip_check(evh_create(cfg, ctx))
+  end subroutine hip_create
+
+end module hydro_dispatch_state
+
+subroutine primitive(w, q)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2) :: w
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_primitive(w, q)
+     return
+  endif
+  if (.not. c_associated(ctx)) call hip_create()
+  if (.not. on_device) then
+     call hip_check(evh_upload_q(ctx, q))
+     on_device = .true.
+  endif
+  call hip_check(evh_primitive(ctx))
+end subroutine primitive
+
+subroutine bound(w)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2) :: w
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_bound(w)
+     return
+  endif
+  call hip_check(evh_bound(ctx))
+end subroutine bound
+
+subroutine hydrostep(w)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2) :: w
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_hydrostep(w)
+  else
+     call hip_check(evh_hydrostep(ctx, dt))
+  endif
+  nsteps_seen = nsteps_seen + 1
+  if (maxsteps .ge. 0 .and. nsteps_seen .gt. maxsteps) tend = t
+  last_step = (tend - t .le. dt)
+end subroutine hydrostep
+
+subroutine compevo(w, evo, s)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2)   :: w
+  real(8), dimension(nw,lx1:mx1,ly1:my1,lz1:mz1)   :: evo
+  real(8), dimension(nw,3,lx1:mx1,ly1:my1,lz1:mz1) :: s
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_compevo(w, evo, s)
+     return
+  endif
+  call hip_check(evh_compevo(ctx, dt))
+end subroutine compevo
+
+subroutine sweep(w, q, evo, s)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nq,lx:mx,ly:my,lz:mz)         :: q
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2)   :: w
+  real(8), dimension(nw,lx1:mx1,ly1:my1,lz1:mz1)   :: evo
+  real(8), dimension(nw,3,lx1:mx1,ly1:my1,lz1:mz1) :: s
+  real(8) :: tsnap_next
+  logical :: need_q
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_sweep(w, q, evo, s)
+     return
+  endif
+  call hip_check(evh_sweep(ctx, dt))
+  ! main.f90:181-195 after this step (t already advanced by updatetime)
+  need_q = last_step
+  tsnap_next = (t - tstart) / (tend - tstart) * nsnap
+  if (tsnap_next .ge. dble(k2s)) then
+     need_q = .true.
+     k2s = k2s + 1
+  endif
+  if ((cut .or. slice .or. bov) .and. tsnap_next .ge. dble(k3s)) then
+     need_q = .true.
+     k3s = k3s + 1
+  endif
+  if (need_q) call hip_check(evh_download_q(ctx, q))
+end subroutine sweep
