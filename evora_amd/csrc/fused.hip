@@ -41,7 +41,21 @@ struct Geo {
   long long n;
   int zchunk;
   int tiles_x, tiles_y, nchunks;
+  int ntiles;  // tiles_x * tiles_y * nchunks
 };
+
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Block b runs on XCD b % 8; giving XCD j the contiguous run of tiles
+// [j*T/8, (j+1)*T/8) (x fastest, then y, then z chunk) keeps the halo
+// columns a block shares with its x and y neighbours in one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  const int j = b % kXcds, i = b / kXcds;
+  const int base = ntiles / kXcds, rem = ntiles % kXcds;
+  const int count = base + (j < rem ? 1 : 0);
+  if (i >= count) return -1;
+  return j * base + (j < rem ? j : rem) + i;
+}
 
 __device__ __forceinline__ int eidx(int x, int y) { return (x + 2) + EW * (y + 2); }
 
@@ -129,8 +143,10 @@ __global__ void __launch_bounds__(NT) k_fused(KParams P, Geo G, const S *__restr
   const int tid = threadIdx.x;
   const bool ring = tid >= NI;
   // tile of this block
-  const int tile = blockIdx.x % (G.tiles_x * G.tiles_y);
-  const int chunk = blockIdx.x / (G.tiles_x * G.tiles_y);
+  const int lin = xcd_tile(blockIdx.x, G.ntiles);
+  if (lin < 0) return;  // padding block (whole block exits before any barrier)
+  const int tile = lin % (G.tiles_x * G.tiles_y);
+  const int chunk = lin / (G.tiles_x * G.tiles_y);
   const int x0 = (tile % G.tiles_x) * TX, y0 = (tile / G.tiles_x) * TY;
   const int z0 = chunk * G.zchunk;
   const int z1 = min(z0 + G.zchunk, G.nz);
@@ -366,13 +382,14 @@ Geo make_geo(const evh_ctx *h) {
   g.tiles_x = (g.nx + TX - 1) / TX;
   g.tiles_y = (g.ny + TY - 1) / TY;
   g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
+  g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
   return g;
 }
 
 template <typename T, typename S, int SOLVER, int LIMITER>
 void launch_fused_t(evh_ctx *h, double dt) {
   const Geo g = make_geo(h);
-  const int blocks = g.tiles_x * g.tiles_y * g.nchunks;
+  const int blocks = kXcds * ((g.ntiles + kXcds - 1) / kXcds);
   hipLaunchKernelGGL((k_fused<T, S, SOLVER, LIMITER>), dim3(blocks), dim3(NT), 0, h->stream, h->kp, g,
                      (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st);
 }
