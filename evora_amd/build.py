@@ -9,6 +9,7 @@ disabled for all device code.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -25,12 +26,15 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-# source -> extra flags
-UNITS = {
-    "api.hip": [],
-    "staged.hip": ["-ffp-contract=off"],
-    "fused.hip": [],
-}
+# (object, source, extra flags)
+UNITS = [
+    ("api.o", "api.hip", []),
+    ("staged.o", "staged.hip", ["-ffp-contract=off"]),
+    ("fused64.o", "fused.hip", ["-DEVH_FUSED_PREC=64"]),
+    ("fused32.o", "fused.hip", ["-DEVH_FUSED_PREC=32"]),
+    ("batch64.o", "batch.hip", ["-DEVH_FUSED_PREC=64"]),
+    ("batch32.o", "batch.hip", ["-DEVH_FUSED_PREC=32"]),
+]
 
 
 def _run(cmd):
@@ -44,14 +48,16 @@ def _run(cmd):
 def build_library(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src, extra in UNITS.items():
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *COMMON, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+    objs, cmds = [], []
+    for obj, src, extra in UNITS:
+        obj = os.path.join(objdir, obj)
+        cmds.append([HIPCC, *COMMON, *extra, "-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd), flush=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
+        list(ex.map(_run, cmds))
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, OUT)

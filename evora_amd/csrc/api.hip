@@ -195,6 +195,23 @@ int validate(const evh_config *c) {
 }  // namespace
 
 namespace evh {
+KParams make_kparams(const evh_config &c) {
+  KParams k;
+  k.nx = c.nx; k.ny = c.ny; k.nz = c.nz;
+  k.boundtype = c.boundtype; k.solver = c.solver; k.limiter = c.limiter;
+  k.secondorder = c.secondorder; k.dualenergy = c.dualenergy;
+  k.onedimension = c.onedimension; k.twodimension = c.twodimension;
+  k.g = c.g; k.g1 = c.g1; k.g1inv = c.g1inv; k.gg1 = c.gg1; k.cfl = c.cfl;
+  k.de_thresh1 = c.de_thresh1;
+  k.dx = c.dx; k.dy = c.dy; k.dz = c.dz;
+  k.dxinv = c.dxinv; k.dyinv = c.dyinv; k.dzinv = c.dzinv;
+  k.mach_fp64 = c.mach_fp64;
+  // Mach M  <=>  kin/e = g (g-1) M^2 / 2  <=>  kin/E = R / (1 + R)
+  const double R = 0.5 * c.g * c.g1 * c.mach_fp64 * c.mach_fp64;
+  k.kin_frac64 = c.mach_fp64 > 0 ? R / (1.0 + R) : -1.0;
+  return k;
+}
+
 hipError_t launch_reset_status(evh_ctx *h, bool keep_next) {
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, h->stream, h->st, keep_next ? 1 : 0);
   return hipGetLastError();
@@ -211,24 +228,15 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
   evh_ctx *h = new (std::nothrow) evh_ctx();
   if (!h) return EVH_ERR_HIP;
   h->cfg = *cfg;
-  KParams &k = h->kp;
-  k.nx = cfg->nx; k.ny = cfg->ny; k.nz = cfg->nz;
-  k.boundtype = cfg->boundtype; k.solver = cfg->solver; k.limiter = cfg->limiter;
-  k.secondorder = cfg->secondorder; k.dualenergy = cfg->dualenergy;
-  k.onedimension = cfg->onedimension; k.twodimension = cfg->twodimension;
-  k.g = cfg->g; k.g1 = cfg->g1; k.g1inv = cfg->g1inv; k.gg1 = cfg->gg1; k.cfl = cfg->cfl;
-  k.de_thresh1 = cfg->de_thresh1;
-  k.dx = cfg->dx; k.dy = cfg->dy; k.dz = cfg->dz;
-  k.dxinv = cfg->dxinv; k.dyinv = cfg->dyinv; k.dzinv = cfg->dzinv;
-  k.mach_fp64 = cfg->mach_fp64;
-  {
-    // Mach M  <=>  kin/e = g (g-1) M^2 / 2  <=>  kin/E = R / (1 + R)
-    const double R = 0.5 * cfg->g * cfg->g1 * cfg->mach_fp64 * cfg->mach_fp64;
-    k.kin_frac64 = cfg->mach_fp64 > 0 ? R / (1.0 + R) : -1.0;
-  }
+  h->kp = make_kparams(*cfg);
   h->grid = Grid{cfg->nx, cfg->ny, cfg->nz};
   h->qelem = mixed(h) ? sizeof(float) : sizeof(double);
   h->mode = fused_supported(*cfg) ? EVH_MODE_FUSED : EVH_MODE_STAGED;
+  // launch shape measured fastest on MI355X for the 512^3 blast (16x12 tiles,
+  // 64-plane chunks; FP64 at 2 waves/SIMD, FP32 at 3)
+  h->tile_ty = 12;
+  h->zchunk = 64;
+  h->wpe_hi = !mixed(h);
 
   auto bail = [&](hipError_t e) {
     int r = hip_fail(h, e);
@@ -552,6 +560,24 @@ int evh_event_elapsed(evh_ctx *h, int a, int b, float *ms) {
 int evh_synchronize(evh_ctx *h) {
   if (!h) return EVH_ERR_STATE;
   return check_status(h);
+}
+
+int evh_set_tuning(evh_ctx *h, int tile_ty, int wpe_hi, int zchunk) {
+  if (!h) return EVH_ERR_STATE;
+  if ((tile_ty != 8 && tile_ty != 12) || (wpe_hi != 0 && wpe_hi != 1) || zchunk < 0)
+    return fail(h, EVH_ERR_CONFIG, "tile_ty must be 8 or 12, wpe_hi 0 or 1, zchunk >= 0");
+  h->tile_ty = tile_ty;
+  h->wpe_hi = wpe_hi != 0;
+  h->zchunk = zchunk;
+  return EVH_OK;
+}
+
+int evh_get_tuning(evh_ctx *h, int *tile_ty, int *wpe_hi, int *zchunk) {
+  if (!h || !tile_ty || !wpe_hi || !zchunk) return EVH_ERR_STATE;
+  *tile_ty = h->tile_ty;
+  *wpe_hi = h->wpe_hi ? 1 : 0;
+  *zchunk = h->zchunk;
+  return EVH_OK;
 }
 
 int evh_set_timing(evh_ctx *h, int on) {

@@ -30,7 +30,9 @@ struct evh_ctx {
   evh::Grid grid;
   int device = 0;
   int mode = EVH_MODE_FUSED;
-  int zchunk = 0;  // z planes per fused block (0: default)
+  int zchunk = 0;   // z planes per fused block (0: default)
+  int tile_ty = 12;    // fused tile height: 8 or 12
+  bool wpe_hi = true;  // register budget for the higher of the two occupancy targets
   hipStream_t stream = nullptr;
   hipEvent_t events[8] = {};
 
@@ -86,7 +88,12 @@ hipError_t launch_sweep(evh_ctx *h, double dt);
 // fused kernels (fused.hip)
 hipError_t launch_cfl(evh_ctx *h);                 // primitive + CFL min over cur q
 hipError_t launch_fused_step(evh_ctx *h, double dt);  // cur -> other, CFL of the result
+hipError_t launch_fused_f64(evh_ctx *h, double dt);
+hipError_t launch_fused_f32(evh_ctx *h, double dt);
 bool fused_supported(const evh_config &c);
+
+// KParams of a configuration (api.hip)
+KParams make_kparams(const evh_config &c);
 
 // transfers and reductions (api.hip)
 hipError_t launch_reset_status(evh_ctx *h, bool keep_next);

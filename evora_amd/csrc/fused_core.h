@@ -1,0 +1,400 @@
+// fused_core.h -- the tile march of the fused hydro step, shared by the
+// single-grid kernel (fused.hip) and the batched persistent kernel
+// (batch.hip).
+//
+// A block owns a 16 x TY column tile of one box and marches up a chunk of z
+// planes, reading q^n once and writing q^{n+1} once; w, evo and s never
+// exist in HBM.  Waves 0..TY/4-1 own the 16*TY interior columns one to one;
+// the last wave (the "ring wave") owns the 2*(16+TY) columns of the first
+// halo ring and loads the second ring and the four corners, so every
+// interior lane does useful work in every stage.  Per plane k:
+//   S1  primitive (comp.f90:26-85) of plane k+1 for every owned column, with
+//       the ghost cells of `bound` (boundary.f90) folded into the load index;
+//       plane k goes to LDS with both halo rings.
+//   S2  limited slopes in x, y, z and the predictor evo (muscl.f90:45-169)
+//       of plane k; the x/y face states go to LDS per face; the interior
+//       lanes solve their z face k-1/2 (riemann.f90) and finish plane k-1
+//       as (q + dtx*(Fx) + dty*(Fy)) + dtz*(Fz), the reference's x, y, z
+//       order (muscl.f90:191-272, integration :287); primitive + the CFL
+//       limit of the new state feed a min reduction for the next hydrostep
+//       (timestep.f90:26-57).
+//   S3  x and y faces of plane k: interior lane (i,j) solves its left and
+//       bottom faces, the ring wave the TY right-edge and 16 top-edge faces.
+//       Each flux overwrites the two states it came from.
+//   S4  q(k) + dtx*(Fx) + dty*(Fy) of plane k, kept for S2 of plane k+1.
+// What a column carries from plane to plane besides its w window (the
+// partial update, the z flux and z state of the face below) lives in
+// lane-private LDS, so no register holds it through the x/y solves.
+// Template parameters: T compute type, S storage type of q (double/double
+// for the FP64 form, float/float for the FP32-mixed form), SOLVER, LIMITER,
+// TY the tile height (4, 8 or 12).
+#pragma once
+
+#include "evh_internal.h"
+#include "physics_fast.h"
+
+namespace evh {
+namespace core {
+
+constexpr int TX = 16;
+
+template <int TY>
+struct TileShape {
+  static constexpr int NI = TX * TY;          // interior lanes
+  static constexpr int NT = NI + 64;          // + the ring wave
+  static constexpr int NR1 = 2 * TX + 2 * TY; // first-ring columns
+  static constexpr int EW = TX + 4;           // extended plane pitch (x from -2 to TX+1)
+  static constexpr int EA = EW * (TY + 4);    // extended plane area
+  static constexpr int NFX = (TX + 1) * TY;   // x faces of a plane
+  static constexpr int NFY = TX * (TY + 1);   // y faces of a plane
+  static_assert(NI % 64 == 0, "interior lanes must fill whole wavefronts");
+  static_assert(NR1 + 4 <= 64, "ring columns and corners must fit one wavefront");
+};
+
+struct Geo {
+  int nx, ny, nz;
+  long long n;
+  int zchunk;
+  int tiles_x, tiles_y, nchunks;
+  int ntiles;  // tiles_x * tiles_y * nchunks
+};
+
+template <int TY>
+__device__ __forceinline__ int eidx(int x, int y) { return (x + 2) + TileShape<TY>::EW * (y + 2); }
+
+// mixed form: FP64 pressure recovery where the flow is fast
+template <typename T>
+struct Prim;
+
+template <>
+struct Prim<double> {
+  __device__ static __forceinline__ int q2w(const KParams &P, double q[6], double w[5], double &rinv) {
+    return fast::q2w(P, q, w, rinv);
+  }
+};
+
+template <>
+struct Prim<float> {
+  // FP32 recovery, except where the subtraction E - kin cancels: there
+  // (kin/E above the Mach threshold) p = g1 (E - kin) is evaluated in FP64.
+  // With dualenergy on, the entropy branch (kin/E > de_thresh1,
+  // comp.f90:61-64) recovers p = S rho**g1 and E = kin + p/g1 in FP64.
+  __device__ static __forceinline__ int q2w(const KParams &P, float q[6], float w[5], float &rinv) {
+    rinv = fm::rcp(q[0]);
+    w[0] = q[0];
+    w[1] = q[1] * rinv;
+    w[2] = q[2] * rinv;
+    w[3] = q[3] * rinv;
+    const float m2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    const float kin = 0.5f * m2 * rinv;
+    const bool fast_flow = kin > float(P.kin_frac64) * q[4];
+    float pr;
+    if (P.dualenergy && kin * fm::rcp(q[4]) > float(P.de_thresh1)) {
+      const double r64 = double(q[0]);
+      const double k64 = 0.5 * (double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]) / r64;
+      const double p64 = double(q[5]) * fm::pow_pos(r64, P.g1);
+      q[4] = float(k64 + p64 * P.g1inv);
+      pr = float(p64);
+    } else {
+      if (fast_flow) {
+        const double k64 = 0.5 * (double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]) / double(q[0]);
+        pr = float(P.g1 * (double(q[4]) - k64));
+      } else {
+        pr = float(P.g1) * (q[4] - kin);
+      }
+      if (P.dualenergy) q[5] = pr * fm::pow_pos(q[0], -float(P.g1));
+    }
+    w[4] = pr;
+    return (pr <= 0.f) ? kErrNegPressure : kOk;
+  }
+};
+
+// raw q at (gx, gy, gz), ghost-mapped (the index rules of `bound`)
+template <typename S>
+__device__ __forceinline__ void fetch_raw(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
+                                          int gz, S raw[6]) {
+  const bool per = P.boundtype == 2;
+  const int cx = gx < -2 ? -2 : (gx > G.nx + 1 ? G.nx + 1 : gx);
+  const int cy = gy < -2 ? -2 : (gy > G.ny + 1 ? G.ny + 1 : gy);
+  const int sx = ghost_src(cx, G.nx, per);
+  const int sy = ghost_src(cy, G.ny, per);
+  const int sz = ghost_src(gz, G.nz, per);
+  const long long c = (long long)sx + (long long)G.nx * ((long long)sy + (long long)G.ny * sz);
+#pragma unroll
+  for (int m = 0; m < 6; m++) raw[m] = q[m * G.n + c];
+}
+
+// primitive of a fetched cell at column gx (for the reflecting wall)
+template <typename S, typename T>
+__device__ __forceinline__ void prim_cell(const KParams &P, const Geo &G, const S raw[6], int gx, T qc[6], T wc[5],
+                                          T &rinv) {
+#pragma unroll
+  for (int m = 0; m < 6; m++) qc[m] = T(raw[m]);
+  Prim<T>::q2w(P, qc, wc, rinv);
+  if (P.boundtype == 3 && gx >= G.nx) {  // reflecting upper x (boundary.f90:57-64)
+    wc[1] = -wc[1];
+    wc[2] = -wc[2];
+    wc[3] = -wc[3];
+  }
+}
+
+// load q at (gx, gy, gz) (ghost-mapped) and recover w
+template <typename S, typename T>
+__device__ __forceinline__ void load_cell(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
+                                          int gz, T qc[6], T wc[5], T &rinv) {
+  S raw[6];
+  fetch_raw(P, G, q, gx, gy, gz, raw);
+  prim_cell(P, G, raw, gx, qc, wc, rinv);
+}
+
+template <typename T, int EA>
+__device__ __forceinline__ void store_w(T (*W)[EA], int slot, const T w[5]) {
+#pragma unroll
+  for (int m = 0; m < 5; m++) W[m][slot] = w[m];
+}
+
+// LDS of one block
+template <typename T, int TY>
+struct Smem {
+  T W[5][TileShape<TY>::EA];    // w of plane k, both halo rings
+  T XB[10][TileShape<TY>::NFX]; // x face (fi, j), fi = 0..TX: [0,5) left state, [5,10) right state; then flux
+  T YB[10][TileShape<TY>::NFY]; // y face (i, fj), fj = 0..TY
+  T CR[17][TileShape<TY>::NI];  // per interior column: partial update [0,6), z flux below [6,12), z state below [12,17)
+};
+
+// One tile `lin` (x fastest, then y, then z chunk) of the box whose q starts
+// at element qoff of qin_all/qout_all.  vmin_dst receives the CFL minimum of
+// the tile's new state; errors carry the box index above bit 40 of the cell.
+template <typename T, typename S, int SOLVER, int LIMITER, int TY>
+__device__ __forceinline__ void march(const KParams P, const Geo G, const S *__restrict__ qin_all,
+                                      S *__restrict__ qout_all, long long qoff, int box, double dt, Status *st,
+                                      unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm) {
+  using TS = TileShape<TY>;
+  constexpr int NI = TS::NI, NR1 = TS::NR1, EW = TS::EW, EA = TS::EA;
+  T(&W)[5][EA] = sm.W;
+  T(&XB)[10][TS::NFX] = sm.XB;
+  T(&YB)[10][TS::NFY] = sm.YB;
+  T(&CR)[17][NI] = sm.CR;
+  const int tid = threadIdx.x;
+  const bool ring = tid >= NI;
+  const int tile = lin % (G.tiles_x * G.tiles_y);
+  const int chunk = lin / (G.tiles_x * G.tiles_y);
+  const int x0 = (tile % G.tiles_x) * TX, y0 = (tile / G.tiles_x) * TY;
+  const int z0 = chunk * G.zchunk;
+  const int z1 = min(z0 + G.zchunk, G.nz);
+  const S *__restrict__ qin = qin_all + qoff;
+  S *__restrict__ qout = qout_all + qoff;
+  const unsigned long long tag = (unsigned long long)box << 40;  // box index above the cell index
+
+  // owned column (interior cell, or first-ring cell for the ring wave); the
+  // ring wave also loads one second-ring cell or first-ring corner per lane
+  int cx, cy, side = -1, r2x = 0, r2y = 0;
+  const int lane = tid - NI;
+  if (!ring) {
+    cx = tid % TX;
+    cy = tid / TX;
+  } else if (lane < TY) {
+    side = 0; cx = -1; cy = lane; r2x = -2; r2y = lane;
+  } else if (lane < 2 * TY) {
+    side = 1; cx = TX; cy = lane - TY; r2x = TX + 1; r2y = cy;
+  } else if (lane < 2 * TY + TX) {
+    side = 2; cx = lane - 2 * TY; cy = -1; r2x = cx; r2y = -2;
+  } else if (lane < NR1) {
+    side = 3; cx = lane - 2 * TY - TX; cy = TY; r2x = cx; r2y = TY + 1;
+  } else {  // lanes NR1..NR1+3 load the first-ring corners, the rest idle
+    const int c = lane - NR1;
+    cx = (c & 1) ? TX : -1;
+    cy = (c & 2) ? TY : -1;
+    r2x = cx;
+    r2y = cy;
+    if (c >= 4) r2x = -3;  // no second-ring job
+  }
+  const bool owns_col = !ring || side >= 0;
+  const int gx = x0 + cx, gy = y0 + cy;
+  const bool inside = !ring && gx < G.nx && gy < G.ny;
+  const int me = eidx<TY>(cx, cy);
+
+  const T dt2 = T(-0.5 * dt);
+  const T dtx = T(dt * P.dxinv), dty = T(dt * P.dyinv), dtz = T(dt * P.dzinv);
+  const T half = T(0.5);
+
+  // FP32 keeps the base q of the update in registers; FP64 reloads it from
+  // cache in S4 (it was read one plane earlier), which saves 24 VGPRs
+  constexpr bool kKeepQ = sizeof(T) == 4;
+  T wm[5], wc[5], wp[5], qn[6], qc[6], tq[6];
+  T rm, rc, rp;
+  double vmin = 1e6;
+  // first Riemann error and first negative pressure of the new state seen by
+  // this lane (code, plane); reported once after the march
+  int err = 0, err_k = 0, nerr = 0, nerr_k = 0;
+
+  load_cell<S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
+  load_cell<S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
+#pragma unroll
+  for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
+
+  for (int k = z0 - 1; k <= z1; k++) {
+    const bool out_plane = k >= z0 && k < z1;
+    // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
+    if (owns_col) {
+      if (kKeepQ) load_cell<S, T>(P, G, qin, gx, gy, k + 1, qn, wp, rp);
+      else load_cell<S, T>(P, G, qin, gx, gy, k + 1, tq, wp, rp);
+    }
+    if (ring && r2x > -3) {
+      T rw[5], rr;
+      load_cell<S, T>(P, G, qin, x0 + r2x, y0 + r2y, k, tq, rw, rr);
+      store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw);
+    }
+    if (owns_col) store_w<T, EA>(W, me, wc);
+    __syncthreads();
+
+    // ---- S2: slopes, predictor and face states of plane k
+    T sx[5], sy[5], sz[5], ev[5];
+    T fz[6];
+    if (owns_col) {
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      sx[m] = fast::slope3<LIMITER>(W[m][me - 1], wc[m], W[m][me + 1]);
+      sy[m] = fast::slope3<LIMITER>(W[m][me - EW], wc[m], W[m][me + EW]);
+      sz[m] = fast::slope3<LIMITER>(wm[m], wc[m], wp[m]);
+    }
+    if (P.secondorder) {
+      T a1[5], a2[5], a3[5];
+      fast::predictor_dir<0>(P, wc, rc, sx, a1);
+      fast::predictor_dir<1>(P, wc, rc, sy, a2);
+      fast::predictor_dir<2>(P, wc, rc, sz, a3);
+      const T ix = T(P.dxinv), iy = T(P.dyinv), iz = T(P.dzinv);
+#pragma unroll
+      for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * ix + a2[m] * iy + a3[m] * iz);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }
+    }
+    // face states (muscl.f90:207-210): w + 0.5 s + evo on the high side, w - 0.5 s + evo on the low side
+    if (!ring) {
+      const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        XB[5 + m][fxl] = wc[m] - half * sx[m] + ev[m];       // right state of the left face
+        XB[m][fxl + 1] = wc[m] + half * sx[m] + ev[m];       // left state of the right face
+        YB[5 + m][fyb] = wc[m] - half * sy[m] + ev[m];
+        YB[m][fyb + TX] = wc[m] + half * sy[m] + ev[m];
+      }
+    } else if (side == 0) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = wc[m] + half * sx[m] + ev[m];
+    } else if (side == 1) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = wc[m] - half * sx[m] + ev[m];
+    } else if (side == 2) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) YB[m][cx] = wc[m] + half * sy[m] + ev[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = wc[m] - half * sy[m] + ev[m];
+    }
+    // z face k-1/2 (interior lanes) and the end of plane k-1; the
+    // scheduling barrier keeps this solve apart from the x/y solves
+    __builtin_amdgcn_sched_barrier(0);
+    if (!ring) {
+      T zl[5], zr[5];
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        zl[m] = wc[m] - half * sz[m] + ev[m];
+        zr[m] = wc[m] + half * sz[m] + ev[m];
+      }
+      if (k >= z0) {
+        T zrp[5], fz[6];
+#pragma unroll
+        for (int m = 0; m < 5; m++) zrp[m] = CR[12 + m][tid];
+        const int e = fast::riemann<SOLVER, 3>(P, zrp, zl, fz);
+        if (e && inside && !err) { err = e; err_k = k; }
+        if (k - 1 >= z0 && inside) {
+          T o[6];
+#pragma unroll
+          for (int m = 0; m < 6; m++) o[m] = CR[m][tid] + dtz * (CR[6 + m][tid] - fz[m]);
+          const long long c = (long long)gx + (long long)G.nx * ((long long)gy + (long long)G.ny * (k - 1));
+#pragma unroll
+          for (int m = 0; m < 6; m++) qout[m * G.n + c] = S(o[m]);
+          T on[6], wn[5], rn;
+#pragma unroll
+          for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
+          const int ep = Prim<T>::q2w(P, on, wn, rn);
+          if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
+          vmin = dmin(vmin, fast::cfl_cell(P, wn, rn));
+        }
+#pragma unroll
+        for (int m = 0; m < 6; m++) CR[6 + m][tid] = fz[m];
+      }
+#pragma unroll
+      for (int m = 0; m < 5; m++) CR[12 + m][tid] = zr[m];
+    }
+    }  // owns_col
+    __syncthreads();
+
+    // ---- S3: x and y faces of plane k
+    if (out_plane) {
+      int fxi = -1, fyi = -1;
+      if (!ring) {
+        fxi = cx + (TX + 1) * cy;
+        fyi = cx + TX * cy;
+      } else {
+        if (lane < TY) fxi = TX + (TX + 1) * lane;          // right-edge x faces
+        else if (lane < TY + TX) fyi = (lane - TY) + TX * TY;  // top-edge y faces
+      }
+      if (fxi >= 0) {
+        T wl[5], wr[5], f[6];
+#pragma unroll
+        for (int m = 0; m < 5; m++) { wl[m] = XB[m][fxi]; wr[m] = XB[5 + m][fxi]; }
+        const int e = fast::riemann<SOLVER, 1>(P, wl, wr, f);
+        if (e && !err && x0 + fxi % (TX + 1) <= G.nx && y0 + fxi / (TX + 1) < G.ny) { err = e; err_k = k; }
+#pragma unroll
+        for (int m = 0; m < 6; m++) XB[m][fxi] = f[m];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (fyi >= 0) {
+        T wl[5], wr[5], f[6];
+#pragma unroll
+        for (int m = 0; m < 5; m++) { wl[m] = YB[m][fyi]; wr[m] = YB[5 + m][fyi]; }
+        const int e = fast::riemann<SOLVER, 2>(P, wl, wr, f);
+        if (e && !err && x0 + fyi % TX < G.nx && y0 + fyi / TX <= G.ny) { err = e; err_k = k; }
+#pragma unroll
+        for (int m = 0; m < 6; m++) YB[m][fyi] = f[m];
+      }
+    }
+    __syncthreads();
+
+    // ---- S4: x/y differences of plane k
+    if (!ring && out_plane) {
+      const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
+      if (!kKeepQ) {  // base q(k) of this column, dual-energy corrected as in S1
+        T wt[5], rt;
+        load_cell<S, T>(P, G, qin, gx, gy, k, qc, wt, rt);
+      }
+      T part[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
+#pragma unroll
+      for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+    }
+    // rotate the column window
+#pragma unroll
+    for (int m = 0; m < 5; m++) { wm[m] = wc[m]; wc[m] = wp[m]; }
+    rm = rc;
+    rc = rp;
+#pragma unroll
+    for (int m = 0; m < 6; m++) qc[m] = qn[m];
+  }
+
+  if (err || nerr) {  // the lane's own column (clamped into the box) locates the error
+    const long long ex = min(max(gx, 0), G.nx - 1), ey = min(max(gy, 0), G.ny - 1);
+    if (err) report(st, err, tag | (unsigned long long)(ex + G.nx * (ey + (long long)G.ny * err_k)));
+    if (nerr) report_next(st, nerr, tag | (unsigned long long)(ex + G.nx * (ey + (long long)G.ny * nerr_k)));
+  }
+  for (int off = 32; off > 0; off >>= 1) vmin = dmin(vmin, __shfl_xor(vmin, off));
+  if ((tid & 63) == 0) atomic_min_pos(vmin_dst, vmin);
+}
+
+
+}  // namespace core
+}  // namespace evh

@@ -36,7 +36,10 @@ EXPORTS = (
     "evh_primitive", "evh_bound", "evh_hydrostep", "evh_compevo", "evh_sweep",
     "evh_step", "evh_run", "evh_sums", "evh_download_w", "evh_download_evo_s",
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
-    "evh_set_timing", "evh_kernel_stats", "evh_device_bytes",
+    "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_device_bytes",
+    "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
+    "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
+    "evh_batch_get_tuning", "evh_batch_autotune", "evh_batch_last_error",
 )
 
 
@@ -90,6 +93,8 @@ def load_library():
             "evh_event_elapsed": [P, C.c_int, C.c_int, C.POINTER(C.c_float)],
             "evh_synchronize": [P],
             "evh_set_timing": [P, C.c_int],
+            "evh_set_tuning": [P, C.c_int, C.c_int, C.c_int],
+            "evh_get_tuning": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "evh_kernel_stats": [P, D, C.POINTER(C.c_longlong)],
         }
         for name, args in sig.items():
@@ -245,6 +250,15 @@ class Hydro:
 
     def synchronize(self):
         self._check(self.L.evh_synchronize(self.h))
+
+    def set_tuning(self, tile_ty: int = 12, wpe_hi: bool = True, zchunk: int = 0):
+        """Launch shape of the fused kernel: tile height, occupancy target, z chunk."""
+        self._check(self.L.evh_set_tuning(self.h, tile_ty, int(bool(wpe_hi)), zchunk))
+
+    def get_tuning(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.evh_get_tuning(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, bool(b.value), c.value
 
     def set_timing(self, on: bool):
         self._check(self.L.evh_set_timing(self.h, int(bool(on))))

@@ -102,6 +102,13 @@ int evh_event_record(evh_ctx *h, int slot);                 /* slot 0..7 */
 int evh_event_elapsed(evh_ctx *h, int a, int b, float *ms); /* syncs on slot b */
 int evh_synchronize(evh_ctx *h);
 
+/* Launch shape of the fused kernel (no counterpart in the reference; set by
+ * the autotuner): tile height 8 or 12 columns, occupancy target (0: the
+ * lower, 1: the higher of the two register budgets compiled per precision),
+ * z planes per block (0: default 128).  Invalid values are rejected. */
+int evh_set_tuning(evh_ctx *h, int tile_ty, int wpe_hi, int zchunk);
+int evh_get_tuning(evh_ctx *h, int *tile_ty, int *wpe_hi, int *zchunk);
+
 /* Kernel timing (no counterpart in the reference; for benchmarks): when on,
  * every fused or staged sweep launch is bracketed by events on the context's
  * stream and its duration is added to a running total at the next sync. */
@@ -110,6 +117,34 @@ int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches);
 
 /* Bytes of device memory the context holds. */
 long long evh_device_bytes(evh_ctx *h);
+
+/* ---- batched form (no counterpart in the reference, whose equivalent is
+ * one run per box): many independent 3-D boxes of uneven sizes, each its own
+ * evora run (own t, dt from its own CFL limit, own error stop), advanced
+ * together.  One step is a device-side dt kernel (hydrostep + updatetime per
+ * box, timestep.f90:26-57,163-169) and one persistent sweep kernel over a
+ * (box, tile) work list; with use_graph the pair is replayed from a hipGraph
+ * and the host does not synchronise between steps.  All boxes share
+ * solver, limiter, precision, secondorder and dualenergy; grids, spacing,
+ * boundtype and cfl may differ. */
+typedef struct evh_batch evh_batch;
+int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out);
+int evh_batch_destroy(evh_batch *b);
+int evh_batch_upload_q(evh_batch *b, int box, const double *q);
+int evh_batch_download_q(evh_batch *b, int box, double *q);
+/* per-box start time and tend (defaults 0 and 1e99); either may be NULL */
+int evh_batch_set_time(evh_batch *b, const double *t, const double *tend);
+/* advance every running box by nsteps steps; ms = device time of the steps */
+int evh_batch_run(evh_batch *b, int nsteps, int use_graph, float *ms);
+/* per-box t, last dt, steps done and go flag; any pointer may be NULL */
+int evh_batch_times(evh_batch *b, double *t, double *dt, int *steps, int *go);
+/* launch shape: tile height 8|12, occupancy target 0|1, z chunk, blocks per CU */
+int evh_batch_set_tuning(evh_batch *b, int tile_ty, int wpe_hi, int zchunk, int blocks_per_cu);
+int evh_batch_get_tuning(evh_batch *b, int *tile_ty, int *wpe_hi, int *zchunk, int *blocks_per_cu);
+/* time every launch shape for trial_steps steps on a snapshot of the state,
+ * keep the fastest, restore the state; the timings are written to log */
+int evh_batch_autotune(evh_batch *b, int trial_steps, char *log, int loglen);
+int evh_batch_last_error(evh_batch *b, int *code, int *box, int cell[3], char *msg, int msglen);
 
 #ifdef __cplusplus
 }

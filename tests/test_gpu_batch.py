@@ -1,0 +1,95 @@
+"""The batched form: every box of a batch must evolve exactly as the same box
+run alone through the single-grid fused path (same per-cell arithmetic, same
+per-box dt), whether the steps run eagerly or replay from a hipGraph."""
+import numpy as np
+import pytest
+
+from evora_amd import hydro, ics
+from evora_amd.batch import Batch
+from evora_amd.params import HydroParams
+
+pytestmark = pytest.mark.gpu
+
+
+def boxes():
+    ps = [HydroParams(nx=24, ny=24, nz=24, solver=2, limiter=1, boundtype=1, cfl=0.4, atomic=True),
+          HydroParams(nx=33, ny=20, nz=17, solver=2, limiter=1, boundtype=2, cfl=0.3, atomic=True),
+          HydroParams(nx=40, ny=36, nz=30, solver=2, limiter=1, boundtype=3, cfl=0.4, atomic=True)]
+    qs = [ics.random_smooth(p, seed=40 + i, amp=0.1) for i, p in enumerate(ps)]
+    return ps, qs
+
+
+def alone(p, q0, nsteps, precision):
+    with hydro.Hydro(p, precision=precision) as h:
+        h.upload_q(q0)
+        t, dt, n = h.run(nsteps)
+        return h.download_q(), t, dt
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.FP32_MIXED])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_single_runs(precision, graph):
+    ps, qs = boxes()
+    with Batch(ps, precision=precision) as b:
+        for i, q in enumerate(qs):
+            b.upload_q(i, q)
+        b.run(5, graph=graph)
+        t, dt, steps, go = b.times()
+        out = [b.download_q(i) for i in range(len(ps))]
+    assert list(steps) == [5, 5, 5]
+    for i, (p, q0) in enumerate(zip(ps, qs)):
+        qa, ta, dta = alone(p, q0, 5, precision)
+        assert t[i] == ta and dt[i] == dta, i
+        assert np.array_equal(out[i], qa), i
+
+
+def test_batch_graph_equals_eager_over_many_steps():
+    ps, qs = boxes()
+    res = []
+    for graph in (False, True):
+        with Batch(ps, precision=hydro.FP32_MIXED) as b:
+            for i, q in enumerate(qs):
+                b.upload_q(i, q)
+            b.run(7, graph=graph)
+            b.run(6, graph=graph)
+            res.append([b.download_q(i) for i in range(len(ps))] + [b.times()[0]])
+    for a, c in zip(res[0], res[1]):
+        assert np.array_equal(a, c)
+
+
+def test_batch_tend_and_error_isolation():
+    ps, qs = boxes()
+    qs[1] = qs[1].copy()
+    qs[1][3, 4, 5, 4] = 0.25 * (qs[1][3, 4, 5, 1:4] ** 2).sum() / qs[1][3, 4, 5, 0]  # E below kinetic
+    with Batch(ps) as b:
+        for i, q in enumerate(qs):
+            b.upload_q(i, q)
+        b.set_time(tend=np.array([1e30, 1e30, 0.004]))
+        with pytest.raises(hydro.EvoraError) as ei:
+            b.run(4)
+        assert ei.value.code == 1 and ei.value.box == 1
+        t, dt, steps, go = b.times()
+        assert steps[1] == 0 and go[1] == 0
+        assert t[2] == pytest.approx(0.004, rel=1e-15) and go[2] == 0  # updatetime's clamp ended box 2
+        q0 = b.download_q(0)
+    qa, ta, _ = alone(ps[0], qs[0], 4, hydro.FP64)
+    assert np.array_equal(q0, qa)
+
+
+def test_batch_autotune_restores_state():
+    ps, qs = boxes()
+    with Batch(ps, precision=hydro.FP32_MIXED) as b:
+        for i, q in enumerate(qs):
+            b.upload_q(i, q)
+        b.run(1)
+        log = b.autotune(trial_steps=1)
+        assert "best:" in log and log.count("ms_per_step") >= 48
+        b.run(3)
+        tuned = [b.download_q(i) for i in range(len(ps))]
+    with Batch(ps, precision=hydro.FP32_MIXED) as b:
+        for i, q in enumerate(qs):
+            b.upload_q(i, q)
+        b.run(4)
+        plain = [b.download_q(i) for i in range(len(ps))]
+    for a, c in zip(tuned, plain):
+        assert np.array_equal(a, c)
