@@ -81,6 +81,42 @@ def run_gpu(n, steps, warmup, precision_name="fp64", solver=2, cfl=0.4):
     return res
 
 
+def run_batch(nbox, steps, warmup, seed=7, log_path=None):
+    """Config 4: nbox independent shock-cloud boxes, edges drawn from
+    {64, 96, 128, 160, 192}, Mach in [3, 30], density contrast in [10, 100],
+    FP32-mixed, CFL 0.6, per-box dt, one persistent launch per step replayed
+    from a hipGraph after the autotuner picked the launch shape."""
+    from evora_amd import hydro, ics
+    from evora_amd.batch import Batch
+
+    spec = ics.ensemble_sizes(nbox, seed=seed)
+    params = [ics.shock_cloud_params(n, cfl=0.6) for n, _, _ in spec]
+    cells = sum(n ** 3 for n, _, _ in spec)
+    log(f"batched: {nbox} boxes, {cells / 1e6:.1f} M cells")
+    with Batch(params, precision=hydro.FP32_MIXED) as b:
+        for i, ((n, mach, chi), p) in enumerate(zip(spec, params)):
+            b.upload_q(i, ics.shock_cloud(p, mach=mach, chi=chi))
+        b.run(warmup, graph=False)
+        tune_log = b.autotune(trial_steps=3)
+        if log_path:
+            with open(log_path, "w") as fh:
+                fh.write(tune_log)
+        b.run(1, graph=True)  # capture
+        t0 = time.perf_counter()
+        ms = b.run(steps, graph=True)
+        wall = time.perf_counter() - t0
+        t, dt, nsteps, go = b.times()
+        shape = b.get_tuning()
+    res = dict(value=cells * steps / wall, ms_per_step=1e3 * wall / steps, device_ms_per_step=ms / steps,
+               boxes=nbox, cells=cells, steps=steps, graph=True,
+               tuning=dict(tile_ty=shape[0], wpe_hi=shape[1], zchunk=shape[2], blocks_per_cu=shape[3]),
+               hbm_frac=BYTES_PER_CELL["fp32_mixed"] * cells / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               min_steps_done=int(nsteps.min()))
+    log(f"batched: {res['value'] / 1e9:.2f} G cell-updates/s, {res['device_ms_per_step']:.3f} ms/step, "
+        f"shape {res['tuning']}")
+    return res
+
+
 def cpu_baseline(n, cores, nsteps=2):
     """The reference's own hot path (oracle/_ref/evref, compiled from the
     reference Fortran at -O3) on `cores` processes, each advancing one z-slab
@@ -134,6 +170,22 @@ def cpu_baseline(n, cores, nsteps=2):
                        f"({cores} slabs, own ghost fill, no halo exchange), slowest slab {slowest:.2f} s")
 
 
+def copy_probe():
+    """A plain device copy of known size (2 GiB read + 2 GiB written, three
+    times): the reference point that scales the FETCH_SIZE / WRITE_SIZE
+    counters of the rocprofv3 traffic runs."""
+    import torch
+
+    n = 2 ** 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {a.numel() * 8} bytes three times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +197,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
+    ap.add_argument("--batch-boxes", type=int, default=48)
+    ap.add_argument("--batch-steps", type=int, default=200)
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the hydro step runs as one rank on one GPU)", file=sys.stderr)
@@ -161,6 +220,10 @@ def main():
         forms["fp32_mixed"] = dict(value=mx["value"], ms_per_step=mx["ms_per_step"], kernel_ms=mx["kernel_ms"],
                                    hbm_frac=ach / HBM_PEAK_GBS, speedup_vs_fp64=mx["value"] / fp64["value"],
                                    mass_drift=(mx["mass"] - fp64["mass"]) / fp64["mass"])
+    if not args.fp64_only and not args.no_batch:
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        forms["batched"] = run_batch(args.batch_boxes, args.batch_steps, 2,
+                                     log_path=os.path.join(ROOT, "gpurun_out", "batch_autotune.log"))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
     if os.path.exists(tpath):
