@@ -4,6 +4,7 @@
 // reference's step-level calls (primitive, bound, hydrostep, compevo, sweep)
 // onto either the staged kernels or the fused one-pass kernel, and converts
 // between the host AoS layout q(6,nx,ny,nz) and the device SoA layout.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -209,6 +210,10 @@ KParams make_kparams(const evh_config &c) {
   // Mach M  <=>  kin/e = g (g-1) M^2 / 2  <=>  kin/E = R / (1 + R)
   const double R = 0.5 * c.g * c.g1 * c.mach_fp64 * c.mach_fp64;
   k.kin_frac64 = c.mach_fp64 > 0 ? R / (1.0 + R) : -1.0;
+  const double delta = 2.0 / 3.0 - c.g1;
+  k.g1_third = std::fabs(delta) <= 1e-6 ? 1 : 0;
+  k.g1_dln2 = delta * 0.69314718055994530942;
+  k.pad_ = 0;
   return k;
 }
 

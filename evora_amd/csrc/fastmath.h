@@ -6,8 +6,9 @@
 // with Newton / Goldschmidt steps to within a few ulp; pow(x, y) for x > 0
 // is exp(y ln x) with ln by the atanh series on the mantissa and exp by a
 // Taylor polynomial after reduction by ln 2.  Relative error of every
-// function is below 1e-15 over the argument range of the hydro step, three
-// orders below the 1e-12 parity bar.  FP32 variants use the single-precision
+// function is below 1e-15 over the argument range of the hydro step (the
+// cube-root powers of g1 below 1e-14), at least two orders below the 1e-12
+// parity bar.  FP32 variants use the single-precision
 // hardware instructions directly.
 #pragma once
 
@@ -96,6 +97,32 @@ __device__ __forceinline__ double exp_fin(double z) {
 
 // x**y for x > 0
 __device__ __forceinline__ double pow_pos(double x, double y) { return exp_fin(y * log_pos(x)); }
+
+// Powers x**(-g1) and x**g1 when g1 = 2/3 - delta with |delta| <= 1e-6 (the
+// single-precision 5/3 of an atomic gas gives delta = 3.97e-8), for
+// positive x = m 2^e with |e| < 200:
+//   y = x**(-1/3): single-precision estimate, then two Newton steps
+//       y += (y/3)(1 - x y^3), each squaring the relative error
+//       (~1e-6 -> ~2e-12 -> rounding);
+//   x**(-g1) = y^2 exp(c),  x**g1 = (x y) exp(-c),  c = delta ln x,
+//   with exp(c) = 1 + c + c^2/2 + c^3/6 (|c| < 1.4e-4, the dropped term is
+//   below 2e-17).  ln x enters only through c, so log2 of the mantissa in
+//   single precision (absolute error ~1e-7, times delta ln 2) is enough:
+//   relative error below 1e-14 (measured 8e-15 over 2^(+-200)).
+// third_root returns y and c; the caller checks the exponent range.
+__device__ __forceinline__ void third_root(double x, int e, double dln2, double &y, double &c) {
+  const double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+  const float l2m = __builtin_amdgcn_logf(float(m));
+  y = double(__builtin_amdgcn_exp2f((float(e) + l2m) * (-1.0f / 3.0f)));
+  y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x, y * y * y, 1.0), y);
+  y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x, y * y * y, 1.0), y);
+  c = dln2 * (double(e) + double(l2m));
+}
+
+// exp(c) - 1 for |c| < 1.4e-4
+__device__ __forceinline__ double expm1_tiny(double c) {
+  return __builtin_fma(__builtin_fma(c, 1.0 / 6.0, 0.5), c * c, c);
+}
 
 // 1/sqrt(x) for positive, normal x (two Newton steps on the rsq estimate)
 __device__ __forceinline__ double rsqrt_pos(double x) {

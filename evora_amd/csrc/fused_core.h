@@ -92,7 +92,7 @@ struct Prim<float> {
     if (P.dualenergy && kin * fm::rcp(q[4]) > float(P.de_thresh1)) {
       const double r64 = double(q[0]);
       const double k64 = 0.5 * (double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]) / r64;
-      const double p64 = double(q[5]) * fm::pow_pos(r64, P.g1);
+      const double p64 = double(q[5]) * fast::pow_g1(P, r64);
       q[4] = float(k64 + p64 * P.g1inv);
       pr = float(p64);
     } else {
@@ -102,7 +102,7 @@ struct Prim<float> {
       } else {
         pr = float(P.g1) * (q[4] - kin);
       }
-      if (P.dualenergy) q[5] = pr * fm::pow_pos(q[0], -float(P.g1));
+      if (P.dualenergy) q[5] = pr * fast::rpow_g1(P, q[0]);
     }
     w[4] = pr;
     return (pr <= 0.f) ? kErrNegPressure : kOk;
@@ -228,8 +228,14 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   // this lane (code, plane); reported once after the march
   int err = 0, err_k = 0, nerr = 0, nerr_k = 0;
 
+  // global loads run one plane ahead: pre holds the raw q of this column's
+  // next plane and pre2 the ring wave's second-ring cell, so that no wave
+  // stalls on memory right after issuing a load
+  S pre[6], pre2[6];
   load_cell<S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
   load_cell<S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
+  fetch_raw(P, G, qin, gx, gy, z0, pre);
+  fetch_raw(P, G, qin, x0 + r2x, y0 + r2y, z0 - 1, pre2);
 #pragma unroll
   for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
 
@@ -237,13 +243,15 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     const bool out_plane = k >= z0 && k < z1;
     // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
     if (owns_col) {
-      if (kKeepQ) load_cell<S, T>(P, G, qin, gx, gy, k + 1, qn, wp, rp);
-      else load_cell<S, T>(P, G, qin, gx, gy, k + 1, tq, wp, rp);
+      if (kKeepQ) prim_cell(P, G, pre, gx, qn, wp, rp);
+      else prim_cell(P, G, pre, gx, tq, wp, rp);
+      fetch_raw(P, G, qin, gx, gy, k + 2, pre);
     }
     if (ring && r2x > -3) {
       T rw[5], rr;
-      load_cell<S, T>(P, G, qin, x0 + r2x, y0 + r2y, k, tq, rw, rr);
+      prim_cell(P, G, pre2, x0 + r2x, tq, rw, rr);
       store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw);
+      fetch_raw(P, G, qin, x0 + r2x, y0 + r2y, k + 1, pre2);
     }
     if (owns_col) store_w<T, EA>(W, me, wc);
     __syncthreads();
@@ -263,35 +271,38 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       fast::predictor_dir<0>(P, wc, rc, sx, a1);
       fast::predictor_dir<1>(P, wc, rc, sy, a2);
       fast::predictor_dir<2>(P, wc, rc, sz, a3);
-      const T ix = T(P.dxinv), iy = T(P.dyinv), iz = T(P.dzinv);
+      const T ex = dt2 * T(P.dxinv), ey = dt2 * T(P.dyinv), ez = dt2 * T(P.dzinv);
 #pragma unroll
-      for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * ix + a2[m] * iy + a3[m] * iz);
+      for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey + a3[m] * ez;
     } else {
 #pragma unroll
       for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }
     }
-    // face states (muscl.f90:207-210): w + 0.5 s + evo on the high side, w - 0.5 s + evo on the low side
+    // face states (muscl.f90:207-210): w + evo + 0.5 s on the high side,
+    // w + evo - 0.5 s on the low side
+#pragma unroll
+    for (int m = 0; m < 5; m++) ev[m] = wc[m] + ev[m];  // ev now holds w + evo
     if (!ring) {
       const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        XB[5 + m][fxl] = wc[m] - half * sx[m] + ev[m];       // right state of the left face
-        XB[m][fxl + 1] = wc[m] + half * sx[m] + ev[m];       // left state of the right face
-        YB[5 + m][fyb] = wc[m] - half * sy[m] + ev[m];
-        YB[m][fyb + TX] = wc[m] + half * sy[m] + ev[m];
+        XB[5 + m][fxl] = ev[m] - half * sx[m];       // right state of the left face
+        XB[m][fxl + 1] = ev[m] + half * sx[m];       // left state of the right face
+        YB[5 + m][fyb] = ev[m] - half * sy[m];
+        YB[m][fyb + TX] = ev[m] + half * sy[m];
       }
     } else if (side == 0) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = wc[m] + half * sx[m] + ev[m];
+      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = ev[m] + half * sx[m];
     } else if (side == 1) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = wc[m] - half * sx[m] + ev[m];
+      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = ev[m] - half * sx[m];
     } else if (side == 2) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) YB[m][cx] = wc[m] + half * sy[m] + ev[m];
+      for (int m = 0; m < 5; m++) YB[m][cx] = ev[m] + half * sy[m];
     } else {
 #pragma unroll
-      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = wc[m] - half * sy[m] + ev[m];
+      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = ev[m] - half * sy[m];
     }
     // z face k-1/2 (interior lanes) and the end of plane k-1; the
     // scheduling barrier keeps this solve apart from the x/y solves
@@ -300,8 +311,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       T zl[5], zr[5];
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        zl[m] = wc[m] - half * sz[m] + ev[m];
-        zr[m] = wc[m] + half * sz[m] + ev[m];
+        zl[m] = ev[m] - half * sz[m];
+        zr[m] = ev[m] + half * sz[m];
       }
       if (k >= z0) {
         T zrp[5], fz[6];

@@ -35,6 +35,9 @@ struct KParams {
   double dx, dy, dz, dxinv, dyinv, dzinv;
   double mach_fp64;
   double kin_frac64;  // FP32-mixed: kin/E above which p is recovered in FP64
+  double g1_dln2;     // (2/3 - g1) ln 2
+  int g1_third;       // |2/3 - g1| <= 1e-6: powers of g1 via the cube root
+  int pad_;
 };
 
 // Device status block: first error (code, linear cell) and min reductions.

@@ -13,6 +13,33 @@
 namespace evh {
 namespace fast {
 
+// rho**(-g1) and rho**g1 of a positive density: the cube-root form of
+// fastmath.h when g1 is close to 2/3 and rho is inside 2^(+-200), the
+// general exp(y ln x) otherwise; single precision uses the hardware
+// exp2/log2.
+__device__ __forceinline__ double rpow_g1(const KParams &P, double x) {
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  if (P.g1_third && e > -200 && e < 200) {
+    double y, c;
+    fm::third_root(x, e, P.g1_dln2, y, c);
+    const double z = y * y;
+    return __builtin_fma(z, fm::expm1_tiny(c), z);
+  }
+  return fm::pow_pos(x, -P.g1);
+}
+__device__ __forceinline__ double pow_g1(const KParams &P, double x) {
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  if (P.g1_third && e > -200 && e < 200) {
+    double y, c;
+    fm::third_root(x, e, P.g1_dln2, y, c);
+    const double v = x * y;
+    return __builtin_fma(v, fm::expm1_tiny(-c), v);
+  }
+  return fm::pow_pos(x, P.g1);
+}
+__device__ __forceinline__ float rpow_g1(const KParams &P, float x) { return fm::pow_pos(x, -float(P.g1)); }
+__device__ __forceinline__ float pow_g1(const KParams &P, float x) { return fm::pow_pos(x, float(P.g1)); }
+
 // comp.f90:26-85 (q2w + pressure); q may be rewritten when dualenergy is on.
 // Returns 1/rho in rinv.
 template <typename T>
@@ -27,11 +54,11 @@ __device__ __forceinline__ int q2w(const KParams &P, T q[6], T w[5], T &rinv) {
   T pr;
   if (P.dualenergy) {
     if (kin * fm::rcp(q[4]) > T(P.de_thresh1)) {  // ratio = kin/E > de_thresh1
-      pr = q[5] * fm::pow_pos(q[0], T(P.g1));
+      pr = q[5] * pow_g1(P, q[0]);
       q[4] = kin + pr * T(P.g1inv);
     } else {
       pr = T(P.g1) * (q[4] - kin);
-      q[5] = pr * fm::rcp(fm::pow_pos(q[0], T(P.g1)));
+      q[5] = pr * rpow_g1(P, q[0]);
     }
   } else {
     pr = T(P.g1) * (q[4] - kin);
@@ -40,14 +67,16 @@ __device__ __forceinline__ int q2w(const KParams &P, T q[6], T w[5], T &rinv) {
   return (pr <= T(0)) ? kErrNegPressure : kOk;
 }
 
-// timestep.f90:39-44 for one cell, from w and 1/rho
+// timestep.f90:39-44 for one cell, from w and 1/rho:
+// min_d dx_d / (|u_d| + a) taken as 1 / max_d (|u_d| + a) / dx_d, one
+// reciprocal instead of three (equal to a few ulp).
 template <typename T>
 __device__ __forceinline__ double cfl_cell(const KParams &P, const T w[5], T rinv) {
   const double a = fm::sqrt_pos(P.g * double(w[4]) * double(rinv));
-  const double vx = P.dx * fm::rcp(fabs(double(w[1])) + a);
-  const double vy = P.dy * fm::rcp(fabs(double(w[2])) + a);
-  const double vz = P.dz * fm::rcp(fabs(double(w[3])) + a);
-  return dmin(dmin(vx, vy), vz);
+  const double sx = (fabs(double(w[1])) + a) * P.dxinv;
+  const double sy = (fabs(double(w[2])) + a) * P.dyinv;
+  const double sz = (fabs(double(w[3])) + a) * P.dzinv;
+  return fm::rcp(dmax(dmax(sx, sy), sz));
 }
 
 // muscl.f90:100-104 with s(5)/rho as s(5)*rinv
@@ -59,19 +88,19 @@ __device__ __forceinline__ void predictor_dir(const KParams &P, const T w[5], T 
   a[D + 1] = a[D + 1] + s[4] * rinv;
 }
 
-// muscl.f90:115-169, one component
+// muscl.f90:115-169, one component.  minmod and monotonized central in
+// the branch-free form max(min(..), min(0, max(..))): it is the smallest
+// candidate when all are positive, the largest when all are negative, and
+// 0 when the one-sided differences disagree in sign, as in the reference.
 template <int LIMITER, typename T>
 __device__ __forceinline__ T slope3(T wl, T wc, T wr) {
   const T dwl = wc - wl, dwr = wr - wc;
   if (LIMITER == 1) {
-    const bool lpos = dwl >= T(0), rpos = dwr >= T(0);
-    if (lpos != rpos) return T(0);
-    return rpos ? dmin(dwl, dwr) : dmax(dwl, dwr);
+    return fmax(fmin(dwl, dwr), fmin(T(0), fmax(dwl, dwr)));
   } else if (LIMITER == 2) {
     const T dwc = (wr - wl) * T(0.5);
-    const bool lpos = dwl >= T(0), rpos = dwr >= T(0);
-    if (lpos != rpos) return T(0);
-    return rpos ? dmin(dmin(T(2) * dwl, dwc), T(2) * dwr) : dmax(dmax(T(2) * dwl, dwc), T(2) * dwr);
+    const T a = T(2) * dwl, b = T(2) * dwr;
+    return fmax(fmin(fmin(a, dwc), b), fmin(T(0), fmax(fmax(a, dwc), b)));
   } else {
     const T pro = dwl * dwr;
     if (pro <= T(0)) return T(0);
@@ -127,13 +156,13 @@ __device__ __forceinline__ int hll(const KParams &P, const T wl[5], const T wr[5
   const T u2l = wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3];
   const T u2r = wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3];
   if (T(0) < sl) {
-    fast::flux<DIR>(P, wl, u2l, wl[4] * fm::rcp(fm::pow_pos(wl[0], T(P.g1))), f);
+    fast::flux<DIR>(P, wl, u2l, wl[4] * rpow_g1(P, wl[0]), f);
   } else if (T(0) > sr) {
-    fast::flux<DIR>(P, wr, u2r, wr[4] * fm::rcp(fm::pow_pos(wr[0], T(P.g1))), f);
+    fast::flux<DIR>(P, wr, u2r, wr[4] * rpow_g1(P, wr[0]), f);
   } else if (sl <= T(0) && T(0) <= sr) {
     T fl[6], fr[6], ql[6], qr[6];
-    const T sql = wl[4] * fm::rcp(fm::pow_pos(wl[0], T(P.g1)));
-    const T sqr = wr[4] * fm::rcp(fm::pow_pos(wr[0], T(P.g1)));
+    const T sql = wl[4] * rpow_g1(P, wl[0]);
+    const T sqr = wr[4] * rpow_g1(P, wr[0]);
     fast::flux<DIR>(P, wl, u2l, sql, fl);
     fast::flux<DIR>(P, wr, u2r, sqr, fr);
     fast::state(P, wl, u2l, sql, ql);
@@ -181,7 +210,7 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
 #pragma unroll
   for (int i = 0; i < 5; i++) w[i] = right ? wr[i] : wl[i];
   const T u2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
-  const T sq = w[4] * fm::pow_pos(w[0], -T(P.g1));  // p / rho**g1
+  const T sq = w[4] * rpow_g1(P, w[0]);  // p / rho**g1
   const T ud = w[DIR];
   const T mu = w[0] * ud;
   const T energy = T(0.5) * w[0] * u2 + w[4] * T(P.g1inv);
