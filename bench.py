@@ -15,8 +15,15 @@ is n^3 cell-updates.  The input is resident in HBM before the timed region.
 
 Printed: one JSON line with the headline number (FP64 fused path) and,
 alongside, the roofline of the dominant kernel (HIP-event timing of each
-sweep launch on the library's stream), and the reference's CPU hot path
-timed on the host's cores on a bounded sample.
+sweep launch on the library's stream), the FP32-mixed and batched forms,
+and the reference's CPU hot path timed on the host's cores on a bounded
+sample.  Unless --no-accuracy is given, each FP32-mixed configuration (the
+blast with HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50
+steps, the cosmological-like field 256^3 for 100 steps, and the batched
+ensemble) is also advanced in FP64 for the same number of steps, and the
+relative L1 errors of rho, p and |v| and the mass and energy drift of both
+forms are reported under forms.*.accuracy_vs_fp64 (outside every timed
+region).
 """
 from __future__ import annotations
 
@@ -41,51 +48,136 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_gpu(n, steps, warmup, precision_name="fp64", solver=2, cfl=0.4):
-    from evora_amd import hydro, ics
+def primitive_np(q, g1, de_thresh1=None):
+    """rho, |v|, p of conserved states q[..., 6] (comp.f90:45-85: p = g1 (E -
+    kin), or p = S rho^g1 where the dual-energy switch takes the entropy)."""
+    rho = q[..., 0]
+    m2 = q[..., 1] ** 2 + q[..., 2] ** 2 + q[..., 3] ** 2
+    kin = 0.5 * m2 / rho
+    p = g1 * (q[..., 4] - kin)
+    if de_thresh1 is not None:
+        ent = kin / q[..., 4] > de_thresh1
+        p = np.where(ent, q[..., 5] * rho ** g1, p)
+    return rho, np.sqrt(m2) / rho, p
 
-    p = ics.blast_params(n, solver=solver, cfl=cfl)
-    d = p.derive()
-    log(f"building {n}^3 blast initial condition")
-    q0 = ics.blast(p)
+
+class L1Error:
+    """Relative L1 error sum|a - b| / sum|b| of rho, p and |v|, accumulated
+    over z-slabs so that 512^3 states need no full-size temporaries."""
+
+    def __init__(self, d):
+        self.g1 = d.g1
+        self.de = d.de_thresh1 if d.dualenergy else None
+        self.num = np.zeros(3)
+        self.den = np.zeros(3)
+
+    def add(self, qa, qb, slab=32):
+        for z in range(0, qa.shape[0], slab):
+            a = primitive_np(qa[z:z + slab], self.g1, self.de)
+            b = primitive_np(qb[z:z + slab], self.g1, self.de)
+            for i in range(3):
+                self.num[i] += np.abs(a[i] - b[i]).sum()
+                self.den[i] += np.abs(b[i]).sum()
+
+    def result(self):
+        r = self.num / np.where(self.den > 0, self.den, 1.0)
+        return dict(l1_rho=float(r[0]), l1_v=float(r[1]), l1_p=float(r[2]))
+
+
+def drift(s_end, s_0):
+    """Relative change of the mass and energy sums (snapshot.f90:169-172)."""
+    return dict(mass=float((s_end[0] - s_0[0]) / s_0[0]), energy=float((s_end[1] - s_0[1]) / s_0[1]))
+
+
+def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False):
+    """Advance q0 warmup + steps steps with the fused path; the `steps`
+    steps are timed with the state resident in HBM."""
+    from evora_amd import hydro
+
     prec = hydro.FP64 if precision_name == "fp64" else hydro.FP32_MIXED
     h = hydro.Hydro(d, precision=prec, mode=hydro.MODE_FUSED)
-    log(f"uploading {q0.nbytes / 1e9:.2f} GB; device bytes {h.device_bytes() / 1e9:.2f} GB")
-    h.upload_q(q0)
-    del q0
-    t = 0.0
-    for _ in range(warmup):
-        t, dt, go = h.step(t, p.tend)
-    h.synchronize()
-    h.set_timing(True)
-    log(f"warmup done ({warmup} steps); timing {steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        t, dt, go = h.step(t, p.tend)
-    h.synchronize()
-    wall = time.perf_counter() - t0
-    kms, launches = h.kernel_stats()
-    sums = h.sums()
-    h.close()
-    cells = d.ncells
+    try:
+        log(f"{precision_name}: uploading {q0.nbytes / 1e9:.2f} GB; device bytes {h.device_bytes() / 1e9:.2f} GB")
+        h.upload_q(q0)
+        sums0 = h.sums()
+        t = 0.0
+        for _ in range(warmup):
+            t, dt, go = h.step(t, tend)
+        h.synchronize()
+        h.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t, dt, go = h.step(t, tend)
+        h.synchronize()
+        wall = time.perf_counter() - t0
+        kms, launches = h.kernel_stats()
+        sums = h.sums()
+        q = h.download_q() if keep_q else None
+    finally:
+        h.close()
     res = dict(
-        value=cells * steps / wall,
+        value=d.ncells * steps / wall,
         ms_per_step=1e3 * wall / steps,
         kernel_ms=kms / max(launches, 1),
         launches=launches,
         t_end=t,
-        mass=float(sums[0] * d.dx * d.dy * d.dz),
+        sums0=sums0,
+        sums=sums,
+        q=q,
     )
     log(f"{precision_name}: {res['value'] / 1e9:.2f} G cell-updates/s, {res['ms_per_step']:.3f} ms/step, "
         f"kernel {res['kernel_ms']:.3f} ms")
     return res
 
 
-def run_batch(nbox, steps, warmup, seed=7, log_path=None):
-    """Config 4: nbox independent shock-cloud boxes, edges drawn from
+def compare_forms(d, r64, r32):
+    """FP32-mixed against FP64 after the same number of steps: L1 errors of
+    rho, p, |v|; each form's drift of the mass and energy sums from its own
+    start; and the FP32-mixed sums at the end relative to the FP64 ones."""
+    e = L1Error(d)
+    e.add(r32["q"], r64["q"])
+    out = e.result()
+    out["drift_fp64"] = drift(r64["sums"], r64["sums0"])
+    out["drift_fp32_mixed"] = drift(r32["sums"], r32["sums0"])
+    out["sums_vs_fp64"] = drift(r32["sums"], r64["sums"])
+    out["fp64_value"] = r64["value"]
+    out["fp32_mixed_value"] = r32["value"]
+    return out
+
+
+def accuracy_case(name, d, q0, steps, control=False):
+    """One configuration advanced `steps` steps in both forms; returns the
+    FP32-mixed errors against FP64.  With `control`, FP64 is also run from
+    the initial state rounded to single precision: its L1 distance from the
+    FP64 run is what the rounding of the input alone grows to in `steps`
+    steps (large for unstable flows such as a shear layer)."""
+    from evora_amd.hydro import EvoraError
+
+    try:
+        r64 = run_gpu(d, q0, steps, 0, "fp64", keep_q=True)
+        r32 = run_gpu(d, q0, steps, 0, "fp32_mixed", keep_q=True)
+        out = compare_forms(d, r64, r32)
+        del r32
+        if control:
+            rc = run_gpu(d, q0.astype(np.float32).astype(np.float64), steps, 0, "fp64", keep_q=True)
+            e = L1Error(d)
+            e.add(rc["q"], r64["q"])
+            out["fp64_from_rounded_input"] = e.result()
+    except EvoraError as e:  # the scheme stopped, as the reference would
+        log(f"accuracy {name}: stopped: {e}")
+        return dict(stopped=str(e), steps=steps)
+    out["steps"] = steps
+    log(f"accuracy {name}: {out}")
+    return out
+
+
+def run_batch(nbox, steps, warmup, seed=7, log_path=None, accuracy=True):
+    """The ensemble: nbox independent shock-cloud boxes, edges drawn from
     {64, 96, 128, 160, 192}, Mach in [3, 30], density contrast in [10, 100],
     FP32-mixed, CFL 0.6, per-box dt, one persistent launch per step replayed
-    from a hipGraph after the autotuner picked the launch shape."""
+    from a hipGraph after the autotuner picked the launch shape.  With
+    `accuracy`, the same boxes are then advanced the same number of steps in
+    FP64 and compared box by box."""
     from evora_amd import hydro, ics
     from evora_amd.batch import Batch
 
@@ -93,9 +185,11 @@ def run_batch(nbox, steps, warmup, seed=7, log_path=None):
     params = [ics.shock_cloud_params(n, cfl=0.6) for n, _, _ in spec]
     cells = sum(n ** 3 for n, _, _ in spec)
     log(f"batched: {nbox} boxes, {cells / 1e6:.1f} M cells")
+    q0 = [ics.shock_cloud(p, mach=mach, chi=chi) for (n, mach, chi), p in zip(spec, params)]
+    acc = None
     with Batch(params, precision=hydro.FP32_MIXED) as b:
-        for i, ((n, mach, chi), p) in enumerate(zip(spec, params)):
-            b.upload_q(i, ics.shock_cloud(p, mach=mach, chi=chi))
+        for i, q in enumerate(q0):
+            b.upload_q(i, q)
         b.run(warmup, graph=False)
         tune_log = b.autotune(trial_steps=3)
         if log_path:
@@ -107,14 +201,48 @@ def run_batch(nbox, steps, warmup, seed=7, log_path=None):
         wall = time.perf_counter() - t0
         t, dt, nsteps, go = b.times()
         shape = b.get_tuning()
+        if accuracy:
+            acc = batch_accuracy(params, q0, b, warmup + 1 + steps)
     res = dict(value=cells * steps / wall, ms_per_step=1e3 * wall / steps, device_ms_per_step=ms / steps,
                boxes=nbox, cells=cells, steps=steps, graph=True,
                tuning=dict(tile_ty=shape[0], wpe_hi=shape[1], zchunk=shape[2], blocks_per_cu=shape[3]),
                hbm_frac=BYTES_PER_CELL["fp32_mixed"] * cells / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
                min_steps_done=int(nsteps.min()))
+    if acc is not None:
+        res["accuracy_vs_fp64"] = acc
     log(f"batched: {res['value'] / 1e9:.2f} G cell-updates/s, {res['device_ms_per_step']:.3f} ms/step, "
         f"shape {res['tuning']}")
     return res
+
+
+def batch_accuracy(params, q0, b32, nsteps):
+    """The FP32-mixed ensemble b32 after nsteps steps against the same boxes
+    advanced nsteps steps in FP64: errors summed over all boxes."""
+    from evora_amd import hydro
+    from evora_amd.batch import Batch
+
+    ds = [p.derive() for p in params]
+    e = L1Error(ds[0])
+    with Batch(params, precision=hydro.FP64) as b64:
+        for i, q in enumerate(q0):
+            b64.upload_q(i, q)
+        b64.run(nsteps, graph=False)
+        s0 = np.zeros(2)
+        s64 = np.zeros(2)
+        s32 = np.zeros(2)
+        for i, d in enumerate(ds):
+            qa, qb = b32.download_q(i), b64.download_q(i)
+            e.add(qa, qb)
+            vol = d.dx * d.dy * d.dz
+            for acc, q in ((s0, q0[i]), (s64, qb), (s32, qa)):
+                acc += vol * np.array([q[..., 0].sum(), q[..., 4].sum()])
+    out = e.result()
+    out["drift_fp64"] = drift(s64, s0)
+    out["drift_fp32_mixed"] = drift(s32, s0)
+    out["sums_vs_fp64"] = drift(s32, s64)
+    out["steps"] = nsteps
+    log(f"accuracy batched: {out}")
+    return out
 
 
 def cpu_baseline(n, cores, nsteps=2):
@@ -198,6 +326,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-mixed against FP64 error and drift runs")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--batch-boxes", type=int, default=48)
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -210,20 +340,40 @@ def main():
               "(the hydro step runs as one rank on one GPU)", file=sys.stderr)
         sys.exit(2)
 
-    fp64 = run_gpu(args.n, args.steps, args.warmup, "fp64", cfl=args.cfl)
+    from evora_amd import ics
+
+    p = ics.blast_params(args.n, solver=2, cfl=args.cfl)
+    d = p.derive()
+    log(f"building {args.n}^3 blast initial condition")
+    q0 = ics.blast(p)
+    acc = not args.no_accuracy and not args.fp64_only
+    fp64 = run_gpu(d, q0, args.steps, args.warmup, "fp64", keep_q=acc)
     achieved = BYTES_PER_CELL["fp64"] * args.n ** 3 / (fp64["kernel_ms"] * 1e-3) / 1e9
     forms = {"fp64": dict(value=fp64["value"], ms_per_step=fp64["ms_per_step"], kernel_ms=fp64["kernel_ms"],
                           hbm_frac=achieved / HBM_PEAK_GBS)}
     if not args.fp64_only:
-        mx = run_gpu(args.n, args.steps, args.warmup, "fp32_mixed", cfl=args.cfl)
+        mx = run_gpu(d, q0, args.steps, args.warmup, "fp32_mixed", keep_q=acc)
         ach = BYTES_PER_CELL["fp32_mixed"] * args.n ** 3 / (mx["kernel_ms"] * 1e-3) / 1e9
         forms["fp32_mixed"] = dict(value=mx["value"], ms_per_step=mx["ms_per_step"], kernel_ms=mx["kernel_ms"],
-                                   hbm_frac=ach / HBM_PEAK_GBS, speedup_vs_fp64=mx["value"] / fp64["value"],
-                                   mass_drift=(mx["mass"] - fp64["mass"]) / fp64["mass"])
+                                   hbm_frac=ach / HBM_PEAK_GBS, speedup_vs_fp64=mx["value"] / fp64["value"])
+        if acc:
+            accuracy = {f"blast{args.n}_hllc": compare_forms(d, fp64, mx)}
+            accuracy[f"blast{args.n}_hllc"]["steps"] = args.warmup + args.steps
+            log(f"accuracy blast hllc: {accuracy[f'blast{args.n}_hllc']}")
+        del fp64["q"], mx["q"]
+        if acc:
+            ph = ics.blast_params(args.n, solver=1, cfl=args.cfl)
+            accuracy[f"blast{args.n}_hll"] = accuracy_case("blast hll", ph.derive(), q0, args.warmup + args.steps)
+            del q0
+            kp = ics.kh_params(256, cfl=0.5)
+            accuracy["kh256"] = accuracy_case("kh", kp.derive(), ics.kelvin_helmholtz(kp), 50, control=True)
+            cp = ics.cosmo_params(256)
+            accuracy["cosmo256"] = accuracy_case("cosmo", cp.derive(), ics.cosmo_field(cp), 100, control=True)
+            forms["fp32_mixed"]["accuracy_vs_fp64"] = accuracy
     if not args.fp64_only and not args.no_batch:
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         forms["batched"] = run_batch(args.batch_boxes, args.batch_steps, 2,
-                                     log_path=os.path.join(ROOT, "gpurun_out", "batch_autotune.log"))
+                                     log_path=os.path.join(ROOT, "gpurun_out", "batch_autotune.log"), accuracy=acc)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
     if os.path.exists(tpath):

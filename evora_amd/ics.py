@@ -191,7 +191,7 @@ def cosmo_field(params: HydroParams, seed=2024, slope=-2.5, rms=0.5, floor=0.05,
     amp = k2 ** (slope / 4.0)
     amp[0, 0, 0] = 0.0
     dk = wk * amp
-    delta = np.fft.irfftn(dk, s=n)
+    delta = np.fft.irfftn(dk, s=n, axes=(0, 1, 2))
     delta *= rms / delta.std()
     dk = np.fft.rfftn(delta)
     rho = np.maximum(1.0 + delta, floor)
@@ -199,7 +199,7 @@ def cosmo_field(params: HydroParams, seed=2024, slope=-2.5, rms=0.5, floor=0.05,
     for K in (KX, KY, KZ):
         vk = 1j * K * dk / k2
         vk[0, 0, 0] = 0.0
-        vel.append(np.fft.irfftn(vk, s=n))
+        vel.append(np.fft.irfftn(vk, s=n, axes=(0, 1, 2)))
     vx, vy, vz = vel
     c = math.sqrt(d.g * temp)
     vrms = np.sqrt(np.mean(vx ** 2 + vy ** 2 + vz ** 2))
