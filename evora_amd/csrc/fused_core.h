@@ -345,32 +345,53 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
 
     // ---- S3: x and y faces of plane k
     if (out_plane) {
-      int fxi = -1, fyi = -1;
       if (!ring) {
-        fxi = cx + (TX + 1) * cy;
-        fyi = cx + TX * cy;
+        // interior lane (i,j): its left x face and bottom y face, as one
+        // straight-line block so that the two solves interleave
+        const int fxi = cx + (TX + 1) * cy, fyi = cx + TX * cy;
+        T xl[5], xr[5], fx[6], yl[5], yr[5], fy[6];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+          xl[m] = XB[m][fxi];
+          xr[m] = XB[5 + m][fxi];
+          yl[m] = YB[m][fyi];
+          yr[m] = YB[5 + m][fyi];
+        }
+        const int ex = fast::riemann<SOLVER, 1>(P, xl, xr, fx);
+        const int ey = fast::riemann<SOLVER, 2>(P, yl, yr, fy);
+        // faces of cells outside the box (partial tiles) do not count
+        const int e = (gy < G.ny && gx <= G.nx && ex) ? ex : ((gx < G.nx && gy <= G.ny) ? ey : 0);
+        const bool take = e && !err;
+        err = take ? e : err;
+        err_k = take ? k : err_k;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+          XB[m][fxi] = fx[m];
+          YB[m][fyi] = fy[m];
+        }
       } else {
-        if (lane < TY) fxi = TX + (TX + 1) * lane;          // right-edge x faces
-        else if (lane < TY + TX) fyi = (lane - TY) + TX * TY;  // top-edge y faces
-      }
-      if (fxi >= 0) {
-        T wl[5], wr[5], f[6];
+        // ring wave: the TY right-edge x faces and the 16 top-edge y faces
+        int fxi = -1, fyi = -1;
+        if (lane < TY) fxi = TX + (TX + 1) * lane;
+        else if (lane < TY + TX) fyi = (lane - TY) + TX * TY;
+        if (fxi >= 0) {
+          T wl[5], wr[5], f[6];
 #pragma unroll
-        for (int m = 0; m < 5; m++) { wl[m] = XB[m][fxi]; wr[m] = XB[5 + m][fxi]; }
-        const int e = fast::riemann<SOLVER, 1>(P, wl, wr, f);
-        if (e && !err && x0 + fxi % (TX + 1) <= G.nx && y0 + fxi / (TX + 1) < G.ny) { err = e; err_k = k; }
+          for (int m = 0; m < 5; m++) { wl[m] = XB[m][fxi]; wr[m] = XB[5 + m][fxi]; }
+          const int e = fast::riemann<SOLVER, 1>(P, wl, wr, f);
+          if (e && !err && x0 + TX <= G.nx && y0 + lane < G.ny) { err = e; err_k = k; }
 #pragma unroll
-        for (int m = 0; m < 6; m++) XB[m][fxi] = f[m];
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      if (fyi >= 0) {
-        T wl[5], wr[5], f[6];
+          for (int m = 0; m < 6; m++) XB[m][fxi] = f[m];
+        }
+        if (fyi >= 0) {
+          T wl[5], wr[5], f[6];
 #pragma unroll
-        for (int m = 0; m < 5; m++) { wl[m] = YB[m][fyi]; wr[m] = YB[5 + m][fyi]; }
-        const int e = fast::riemann<SOLVER, 2>(P, wl, wr, f);
-        if (e && !err && x0 + fyi % TX < G.nx && y0 + fyi / TX <= G.ny) { err = e; err_k = k; }
+          for (int m = 0; m < 5; m++) { wl[m] = YB[m][fyi]; wr[m] = YB[5 + m][fyi]; }
+          const int e = fast::riemann<SOLVER, 2>(P, wl, wr, f);
+          if (e && !err && x0 + lane - TY < G.nx && y0 + TY <= G.ny) { err = e; err_k = k; }
 #pragma unroll
-        for (int m = 0; m < 6; m++) YB[m][fyi] = f[m];
+          for (int m = 0; m < 6; m++) YB[m][fyi] = f[m];
+        }
       }
     }
     __syncthreads();

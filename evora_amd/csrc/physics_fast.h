@@ -144,10 +144,14 @@ __device__ __forceinline__ void statex(const KParams &P, const T w[5], T rinv, T
   qx[5] = qx[0] * sq * rinv;  // qx1 * p / rho**g
 }
 
-// hll (riemann.f90:47-92)
+// hll (riemann.f90:47-92), branch-free: both sides' flux and state are
+// formed and the region (0 < sl: left, 0 > sr: right, else the HLL
+// average) is selected, so that the solver is one basic block and
+// independent solves can be interleaved by the compiler.  The error codes
+// are the reference's: non-positive pressure first, then no region.
 template <int DIR, typename T>
 __device__ __forceinline__ int hll(const KParams &P, const T wl[5], const T wr[5], T f[6]) {
-  if (wl[4] <= T(0) || wr[4] <= T(0)) return kErrRiemannPressure;
+  int code = (wl[4] <= T(0) || wr[4] <= T(0)) ? kErrRiemannPressure : kOk;
   const T ril = fm::rcp(wl[0]), rir = fm::rcp(wr[0]);
   const T al = fm::sqrt_pos(T(P.g) * wl[4] * ril);
   const T ar = fm::sqrt_pos(T(P.g) * wr[4] * rir);
@@ -155,26 +159,25 @@ __device__ __forceinline__ int hll(const KParams &P, const T wl[5], const T wr[5
   const T sr = dmax(wl[DIR] + al, wr[DIR] + ar);
   const T u2l = wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3];
   const T u2r = wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3];
-  if (T(0) < sl) {
-    fast::flux<DIR>(P, wl, u2l, wl[4] * rpow_g1(P, wl[0]), f);
-  } else if (T(0) > sr) {
-    fast::flux<DIR>(P, wr, u2r, wr[4] * rpow_g1(P, wr[0]), f);
-  } else if (sl <= T(0) && T(0) <= sr) {
-    T fl[6], fr[6], ql[6], qr[6];
-    const T sql = wl[4] * rpow_g1(P, wl[0]);
-    const T sqr = wr[4] * rpow_g1(P, wr[0]);
-    fast::flux<DIR>(P, wl, u2l, sql, fl);
-    fast::flux<DIR>(P, wr, u2r, sqr, fr);
-    fast::state(P, wl, u2l, sql, ql);
-    fast::state(P, wr, u2r, sqr, qr);
-    const T inv = fm::rcp(sr - sl);
-    const T slsr = sl * sr;
+  const bool left = T(0) < sl;
+  const bool right = T(0) > sr;
+  const bool mid = sl <= T(0) && T(0) <= sr;
+  if (!left && !right && !mid && !code) code = kErrHll;
+  T fl[6], fr[6], ql[6], qr[6];
+  const T sql = wl[4] * rpow_g1(P, wl[0]);
+  const T sqr = wr[4] * rpow_g1(P, wr[0]);
+  fast::flux<DIR>(P, wl, u2l, sql, fl);
+  fast::flux<DIR>(P, wr, u2r, sqr, fr);
+  fast::state(P, wl, u2l, sql, ql);
+  fast::state(P, wr, u2r, sqr, qr);
+  const T inv = fm::rcp(sr - sl);
+  const T slsr = sl * sr;
 #pragma unroll
-    for (int i = 0; i < 6; i++) f[i] = (sr * fl[i] - sl * fr[i] + slsr * (qr[i] - ql[i])) * inv;
-  } else {
-    return kErrHll;
+  for (int i = 0; i < 6; i++) {
+    const T fm_ = (sr * fl[i] - sl * fr[i] + slsr * (qr[i] - ql[i])) * inv;
+    f[i] = left ? fl[i] : (right ? fr[i] : fm_);
   }
-  return kOk;
+  return code;
 }
 
 // sound speed sqrt(g p / rho) = g p / sqrt(g p rho)
@@ -191,7 +194,9 @@ __device__ __forceinline__ T sound(const KParams &P, T rho, T p) {
 // which equals the reference expression and needs no 1/rho or 1/(s-u).
 template <int DIR, typename T>
 __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[5], T f[6]) {
-  if (wl[4] <= T(0) || wr[4] <= T(0)) return kErrRiemannPressure;
+  // branch-free, like hll: the region is an index, the star correction is
+  // always formed and kept only in the star regions
+  int code = (wl[4] <= T(0) || wr[4] <= T(0)) ? kErrRiemannPressure : kOk;
   const T al = sound(P, wl[0], wl[4]);
   const T ar = sound(P, wr[0], wr[4]);
   const T sl = dmin(wl[DIR] - al, wr[DIR] - ar);
@@ -199,13 +204,12 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   const T facl = wl[0] * (sl - wl[DIR]);
   const T facr = wr[0] * (sr - wr[DIR]);
   const T sm = (wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR]) * fm::rcp(facl - facr);
-  int br;
-  if (T(0) < sl) br = 0;
-  else if (T(0) > sr) br = 1;
-  else if (sl < T(0) && T(0) < sm) br = 2;
-  else if (sm <= T(0) && T(0) < sr) br = 3;
-  else return kErrHllc;
-  const bool right = (br & 1) != 0;
+  const int br = (T(0) < sl) ? 0
+               : (T(0) > sr) ? 1
+               : (sl < T(0) && T(0) < sm) ? 2
+               : (sm <= T(0) && T(0) < sr) ? 3 : -1;
+  if (br < 0 && !code) code = kErrHllc;
+  const bool right = br == 1 || br == 3;
   T w[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) w[i] = right ? wr[i] : wl[i];
@@ -219,21 +223,20 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   for (int i = 1; i <= 3; i++) f[i] = (i == DIR) ? (mu * ud + w[4]) : (mu * w[i]);
   f[4] = ud * (energy + w[4]);  // ud (0.5 rho u2 + gg1 p) with gg1 = 1 + 1/g1
   f[5] = ud * sq;
-  if (br >= 2) {
-    const T s = right ? sr : sl;
-    const T isd = fm::rcp(s - sm);
-    const T r = (s - ud) * isd;
-    const T qx0 = w[0] * r;
-    T dq[6];
-    dq[0] = qx0 - w[0];
+  const T s = right ? sr : sl;
+  const T isd = fm::rcp(s - sm);
+  const T r = (s - ud) * isd;
+  const T qx0 = w[0] * r;
+  T dq[6];
+  dq[0] = qx0 - w[0];
 #pragma unroll
-    for (int i = 1; i <= 3; i++) dq[i] = ((i == DIR) ? sm : w[i]) * qx0 - w[0] * w[i];
-    dq[4] = r * energy + (sm - ud) * (qx0 * sm + w[4] * isd) - energy;
-    dq[5] = r * sq - sq;
+  for (int i = 1; i <= 3; i++) dq[i] = ((i == DIR) ? sm : w[i]) * qx0 - w[0] * w[i];
+  dq[4] = r * energy + (sm - ud) * (qx0 * sm + w[4] * isd) - energy;
+  dq[5] = r * sq - sq;
+  const bool star = br >= 2;
 #pragma unroll
-    for (int i = 0; i < 6; i++) f[i] = f[i] + s * dq[i];
-  }
-  return kOk;
+  for (int i = 0; i < 6; i++) f[i] = star ? f[i] + s * dq[i] : f[i];
+  return code;
 }
 
 template <int SOLVER, int DIR, typename T>
