@@ -218,9 +218,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   const T dtx = T(dt * P.dxinv), dty = T(dt * P.dyinv), dtz = T(dt * P.dzinv);
   const T half = T(0.5);
 
-  // FP32 keeps the base q of the update in registers; FP64 reloads it from
-  // cache in S4 (it was read one plane earlier), which saves 24 VGPRs
-  constexpr bool kKeepQ = sizeof(T) == 4;
+  // qc, qn: the (dual-energy corrected) q of planes k and k+1, the base of
+  // the update, kept in registers so that q is read from HBM once
   T wm[5], wc[5], wp[5], qn[6], qc[6], tq[6];
   T rm, rc, rp;
   double vmin = 1e6;
@@ -243,8 +242,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     const bool out_plane = k >= z0 && k < z1;
     // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
     if (owns_col) {
-      if (kKeepQ) prim_cell(P, G, pre, gx, qn, wp, rp);
-      else prim_cell(P, G, pre, gx, tq, wp, rp);
+      prim_cell(P, G, pre, gx, qn, wp, rp);
       fetch_raw(P, G, qin, gx, gy, k + 2, pre);
     }
     if (ring && r2x > -3) {
@@ -399,10 +397,6 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     // ---- S4: x/y differences of plane k
     if (!ring && out_plane) {
       const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
-      if (!kKeepQ) {  // base q(k) of this column, dual-energy corrected as in S1
-        T wt[5], rt;
-        load_cell<S, T>(P, G, qin, gx, gy, k, qc, wt, rt);
-      }
       T part[6];
 #pragma unroll
       for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
