@@ -15,9 +15,10 @@ is n^3 cell-updates.  The input is resident in HBM before the timed region.
 
 Printed: one JSON line with the headline number (FP64 fused path) and,
 alongside, the roofline of the dominant kernel (HIP-event timing of each
-sweep launch on the library's stream), the FP32-mixed and batched forms,
-and the reference's CPU hot path timed on the host's cores on a bounded
-sample.  Unless --no-accuracy is given, each FP32-mixed configuration (the
+sweep launch on the library's stream), the FP64 fused step in its exact form
+(the reference's own arithmetic, bit-identical results), the FP32-mixed and
+batched forms, and the reference's CPU hot path timed on the host's cores on
+a bounded sample.  Unless --no-accuracy is given, each FP32-mixed configuration (the
 blast with HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50
 steps, the cosmological-like field 256^3 for 100 steps, and the batched
 ensemble) is also advanced in FP64 for the same number of steps, and the
@@ -89,13 +90,14 @@ def drift(s_end, s_0):
     return dict(mass=float((s_end[0] - s_0[0]) / s_0[0]), energy=float((s_end[1] - s_0[1]) / s_0[1]))
 
 
-def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False):
-    """Advance q0 warmup + steps steps with the fused path; the `steps`
-    steps are timed with the state resident in HBM."""
+def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False, exact=False):
+    """Advance q0 warmup + steps steps with the fused path (exact: the FP64
+    fused step with the reference's own arithmetic); the `steps` steps are
+    timed with the state resident in HBM."""
     from evora_amd import hydro
 
     prec = hydro.FP64 if precision_name == "fp64" else hydro.FP32_MIXED
-    h = hydro.Hydro(d, precision=prec, mode=hydro.MODE_FUSED)
+    h = hydro.Hydro(d, precision=prec, mode=hydro.MODE_FUSED_EXACT if exact else hydro.MODE_FUSED)
     try:
         log(f"{precision_name}: uploading {q0.nbytes / 1e9:.2f} GB; device bytes {h.device_bytes() / 1e9:.2f} GB")
         h.upload_q(q0)
@@ -351,6 +353,10 @@ def main():
     achieved = BYTES_PER_CELL["fp64"] * args.n ** 3 / (fp64["kernel_ms"] * 1e-3) / 1e9
     forms = {"fp64": dict(value=fp64["value"], ms_per_step=fp64["ms_per_step"], kernel_ms=fp64["kernel_ms"],
                           hbm_frac=achieved / HBM_PEAK_GBS)}
+    ex = run_gpu(d, q0, args.steps, args.warmup, "fp64", exact=True)
+    forms["fp64_exact"] = dict(value=ex["value"], ms_per_step=ex["ms_per_step"], kernel_ms=ex["kernel_ms"],
+                               hbm_frac=BYTES_PER_CELL["fp64"] * args.n ** 3 / (ex["kernel_ms"] * 1e-3) / 1e9
+                               / HBM_PEAK_GBS)
     if not args.fp64_only:
         mx = run_gpu(d, q0, args.steps, args.warmup, "fp32_mixed", keep_q=acc)
         ach = BYTES_PER_CELL["fp32_mixed"] * args.n ** 3 / (mx["kernel_ms"] * 1e-3) / 1e9

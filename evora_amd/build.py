@@ -32,6 +32,7 @@ UNITS = [
     ("staged.o", "staged.hip", ["-ffp-contract=off"]),
     ("fused64.o", "fused.hip", ["-DEVH_FUSED_PREC=64"]),
     ("fused32.o", "fused.hip", ["-DEVH_FUSED_PREC=32"]),
+    ("fused64x.o", "fused.hip", ["-DEVH_FUSED_PREC=64", "-DEVH_FUSED_EXACT=1", "-ffp-contract=off"]),
     ("batch64.o", "batch.hip", ["-DEVH_FUSED_PREC=64"]),
     ("batch32.o", "batch.hip", ["-DEVH_FUSED_PREC=32"]),
 ]

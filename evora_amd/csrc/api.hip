@@ -296,9 +296,9 @@ int evh_destroy(evh_ctx *h) {
 
 int evh_set_mode(evh_ctx *h, int mode) {
   if (!h) return EVH_ERR_STATE;
-  if (mode == EVH_MODE_FUSED && !fused_supported(h->cfg)) return fail(h, EVH_ERR_CONFIG, "fused needs a 3-D FP64 grid");
-  if (mode == EVH_MODE_STAGED && mixed(h)) return fail(h, EVH_ERR_CONFIG, "staged kernels are FP64");
-  if (mode != EVH_MODE_FUSED && mode != EVH_MODE_STAGED) return fail(h, EVH_ERR_CONFIG);
+  if (mode != EVH_MODE_FUSED && mode != EVH_MODE_STAGED && mode != EVH_MODE_FUSED_EXACT) return fail(h, EVH_ERR_CONFIG);
+  if (mode != EVH_MODE_STAGED && !fused_supported(h->cfg)) return fail(h, EVH_ERR_CONFIG, "fused needs a 3-D grid");
+  if (mode != EVH_MODE_FUSED && mixed(h)) return fail(h, EVH_ERR_CONFIG, "staged and exact kernels are FP64");
   h->mode = mode;
   h->v_valid = false;
   h->w_valid = false;

@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu
     const int b = box_of(boxes, nbox, item);
     if (bs.active[b]) {
       const BoxDev &bx = boxes[b];
-      core::march<T, S, SOLVER, LIMITER, TY>(bx.P, bx.G, qin, qout, bx.qoff, b, bs.dt[b], &bs.st[b],
+      core::march<T, S, SOLVER, LIMITER, TY, false>(bx.P, bx.G, qin, qout, bx.qoff, b, bs.dt[b], &bs.st[b],
                                              &bs.st[b].vmin_bits, item - bx.tile0, sm);
     }
     __syncthreads();  // s_item is rewritten only after every lane has used it

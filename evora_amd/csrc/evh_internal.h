@@ -90,6 +90,7 @@ hipError_t launch_cfl(evh_ctx *h);                 // primitive + CFL min over c
 hipError_t launch_fused_step(evh_ctx *h, double dt);  // cur -> other, CFL of the result
 hipError_t launch_fused_f64(evh_ctx *h, double dt);
 hipError_t launch_fused_f32(evh_ctx *h, double dt);
+hipError_t launch_fused_exact(evh_ctx *h, double dt);  // FP64, reference arithmetic
 bool fused_supported(const evh_config &c);
 
 // KParams of a configuration (api.hip)

@@ -2,12 +2,22 @@
 //
 // k_fused runs the tile march of fused_core.h over one grid: one block per
 // (16 x TY tile, z chunk), with the tiles of each XCD contiguous.  k_cfl
-// gives the first step's CFL limit.
+// gives the first step's CFL limit.  The file is compiled three times:
+// EVH_FUSED_PREC=64 (FP64, fast arithmetic), EVH_FUSED_PREC=32 (FP32-mixed)
+// and EVH_FUSED_EXACT=1 (FP64 with the reference's arithmetic, compiled
+// without FMA contraction).
 #include "fused_core.h"
 
 #ifndef EVH_FUSED_PREC
 #define EVH_FUSED_PREC 64
 #endif
+#ifndef EVH_FUSED_EXACT
+#define EVH_FUSED_EXACT 0
+#endif
+#if EVH_FUSED_EXACT && EVH_FUSED_PREC != 64
+#error "the exact form is FP64"
+#endif
+constexpr bool kExact = EVH_FUSED_EXACT != 0;
 
 namespace evh {
 namespace {
@@ -35,7 +45,7 @@ __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu
   __shared__ core::Smem<T, TY> sm;
   const int lin = xcd_tile(blockIdx.x, G.ntiles);
   if (lin < 0) return;  // padding block (whole block exits before any barrier)
-  core::march<T, S, SOLVER, LIMITER, TY>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm);
+  core::march<T, S, SOLVER, LIMITER, TY, kExact>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm);
 }
 
 // primitive + CFL min over q without materialising w (first step, after
@@ -107,7 +117,7 @@ hipError_t launch_fused_prec(evh_ctx *h, double dt) {
 
 }  // namespace
 
-#if EVH_FUSED_PREC != 32
+#if EVH_FUSED_PREC == 64 && !EVH_FUSED_EXACT
 bool fused_supported(const evh_config &c) { return c.nx >= 2 && c.ny >= 2 && c.nz >= 2; }
 
 hipError_t launch_cfl(evh_ctx *h) {
@@ -124,13 +134,16 @@ hipError_t launch_cfl(evh_ctx *h) {
 }
 #endif
 
-#if EVH_FUSED_PREC == 32
+#if EVH_FUSED_EXACT
+hipError_t launch_fused_exact(evh_ctx *h, double dt) { return launch_fused_prec<double, double>(h, dt); }
+#elif EVH_FUSED_PREC == 32
 hipError_t launch_fused_f32(evh_ctx *h, double dt) { return launch_fused_prec<float, float>(h, dt); }
 #else
 hipError_t launch_fused_f64(evh_ctx *h, double dt) { return launch_fused_prec<double, double>(h, dt); }
 
 hipError_t launch_fused_step(evh_ctx *h, double dt) {
   if (h->cfg.precision == EVH_FP32_MIXED) return launch_fused_f32(h, dt);
+  if (h->mode == EVH_MODE_FUSED_EXACT) return launch_fused_exact(h, dt);
   return launch_fused_f64(h, dt);
 }
 #endif

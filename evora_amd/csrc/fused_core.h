@@ -109,6 +109,175 @@ struct Prim<float> {
   }
 };
 
+// Arithmetic of the march.  Phys<false> is the fast form (physics_fast.h:
+// refined hardware reciprocals and roots, cube-root powers, FMA contraction).
+// Phys<true> is the reference's own arithmetic (physics.h: IEEE division
+// and square root, the reference's operation order; its translation unit is
+// compiled without FMA contraction), so that every face state, flux and
+// update is the one the reference forms, bit for bit except where the
+// entropy takes a power (device pow against the host libm).
+template <bool EXACT>
+struct Phys;
+
+template <>
+struct Phys<false> {
+  template <typename T>
+  __device__ static __forceinline__ int q2w(const KParams &P, T q[6], T w[5], T &rinv) {
+    return Prim<T>::q2w(P, q, w, rinv);
+  }
+  template <int LIMITER, typename T>
+  __device__ static __forceinline__ T slope3(T wl, T wc, T wr) {
+    return fast::slope3<LIMITER>(wl, wc, wr);
+  }
+  template <int D, typename T>
+  __device__ static __forceinline__ void predictor_dir(const KParams &P, const T w[5], T rinv, const T s[5], T a[5]) {
+    fast::predictor_dir<D>(P, w, rinv, s, a);
+  }
+  template <int SOLVER, int DIR, typename T>
+  __device__ static __forceinline__ int riemann(const KParams &P, const T wl[5], const T wr[5], T f[6]) {
+    return fast::riemann<SOLVER, DIR>(P, wl, wr, f);
+  }
+  template <typename T>
+  __device__ static __forceinline__ double cfl_cell(const KParams &P, const T w[5], T rinv) {
+    return fast::cfl_cell(P, w, rinv);
+  }
+};
+
+// The reference's HLL and HLLC (riemann.f90:47-153) with its expressions
+// and operation order (physics.h), written branch-free like the fast forms:
+// the side is selected first and one flux / state / star state is formed.
+// The entropy components take p / rho**g1 from the cube-root power
+// (fastmath.h); they are the only ones a power enters, and the device and
+// host powers differ in the last bits in any case.
+namespace exact_bf {
+template <int DIR>
+__device__ __forceinline__ void flux(const KParams &P, const double w[5], double sq, double f[6]) {
+  const double u2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+  const double ud = w[DIR];
+  f[0] = w[0] * ud;
+#pragma unroll
+  for (int i = 1; i <= 3; i++) f[i] = (i == DIR) ? (w[0] * (ud * ud) + w[4]) : (w[0] * ud * w[i]);
+  f[4] = ud * (0.5 * w[0] * u2 + P.gg1 * w[4]);
+  f[5] = ud * sq;
+}
+__device__ __forceinline__ void state(const KParams &P, const double w[5], double sq, double q[6]) {
+  q[0] = w[0];
+  q[1] = w[0] * w[1];
+  q[2] = w[0] * w[2];
+  q[3] = w[0] * w[3];
+  const double u2 = (w[1] * w[1]) + (w[2] * w[2]) + (w[3] * w[3]);
+  q[4] = 0.5 * w[0] * u2 + w[4] / P.g1;
+  q[5] = sq;
+}
+template <int DIR>
+__device__ __forceinline__ int hll(const KParams &P, const double wl[5], const double wr[5], double f[6]) {
+  int code = (wl[4] <= 0.0 || wr[4] <= 0.0) ? kErrRiemannPressure : kOk;
+  const double al = sqrt(P.g * wl[4] / wl[0]);
+  const double ar = sqrt(P.g * wr[4] / wr[0]);
+  const double sl = dmin(wl[DIR] - al, wr[DIR] - ar);
+  const double sr = dmax(wl[DIR] + al, wr[DIR] + ar);
+  const bool left = 0.0 < sl, right = 0.0 > sr, mid = sl <= 0.0 && 0.0 <= sr;
+  if (!left && !right && !mid && !code) code = kErrHll;
+  const double sql = wl[4] * fast::rpow_g1(P, wl[0]);
+  const double sqr = wr[4] * fast::rpow_g1(P, wr[0]);
+  double fl[6], fr[6], ql[6], qr[6];
+  flux<DIR>(P, wl, sql, fl);
+  flux<DIR>(P, wr, sqr, fr);
+  state(P, wl, sql, ql);
+  state(P, wr, sqr, qr);
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const double fm_ = (sr * fl[i] - sl * fr[i] + sl * sr * (qr[i] - ql[i])) / (sr - sl);
+    f[i] = left ? fl[i] : (right ? fr[i] : fm_);
+  }
+  return code;
+}
+template <int DIR>
+__device__ __forceinline__ int hllc(const KParams &P, const double wl[5], const double wr[5], double f[6]) {
+  int code = (wl[4] <= 0.0 || wr[4] <= 0.0) ? kErrRiemannPressure : kOk;
+  const double al = sqrt(P.g * wl[4] / wl[0]);
+  const double ar = sqrt(P.g * wr[4] / wr[0]);
+  const double sl = dmin(wl[DIR] - al, wr[DIR] - ar);
+  const double sr = dmax(wl[DIR] + al, wr[DIR] + ar);
+  const double facl = wl[0] * (sl - wl[DIR]);
+  const double facr = wr[0] * (sr - wr[DIR]);
+  const double sm = (wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR]) / (facl - facr);
+  const int br = (0.0 < sl) ? 0
+               : (0.0 > sr) ? 1
+               : (sl < 0.0 && 0.0 < sm) ? 2
+               : (sm <= 0.0 && 0.0 < sr) ? 3 : -1;
+  if (br < 0 && !code) code = kErrHllc;
+  const bool right = br == 1 || br == 3;
+  double w[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) w[i] = right ? wr[i] : wl[i];
+  const double sq = w[4] * fast::rpow_g1(P, w[0]);
+  flux<DIR>(P, w, sq, f);
+  // star state (statex, riemann.f90:241-288) and f + s (qx - q)
+  const double s = right ? sr : sl;
+  const double ud = w[DIR];
+  double q[6], qx[6];
+  state(P, w, sq, q);
+  qx[0] = w[0] * (s - ud) / (s - sm);
+#pragma unroll
+  for (int i = 1; i <= 3; i++) qx[i] = ((i == DIR) ? sm : w[i]) * qx[0];
+  const double energy = 0.5 * w[0] * ((w[1] * w[1]) + (w[2] * w[2]) + (w[3] * w[3])) + w[4] / P.g1;
+  qx[4] = qx[0] * (energy / w[0] + (sm - ud) * (sm + (w[4] / (w[0] * (s - ud)))));
+  qx[5] = qx[0] * (sq / w[0]);
+  const bool star = br >= 2;
+#pragma unroll
+  for (int i = 0; i < 6; i++) f[i] = star ? f[i] + s * (qx[i] - q[i]) : f[i];
+  return code;
+}
+}  // namespace exact_bf
+
+template <>
+struct Phys<true> {
+  // q2w + pressure (comp.f90:26-85) with the reference's IEEE arithmetic;
+  // the dual-energy branch takes rho**g1 from the cube-root power
+  __device__ static __forceinline__ int q2w(const KParams &P, double q[6], double w[5], double &rinv) {
+    rinv = 0.0;  // the exact predictor divides by rho itself
+    const double densinv = 1.0 / q[0];
+    w[0] = q[0];
+    w[1] = q[1] * densinv;
+    w[2] = q[2] * densinv;
+    w[3] = q[3] * densinv;
+    const double m2 = (q[1] * q[1]) + (q[2] * q[2]) + (q[3] * q[3]);
+    const double kin = 0.5 * m2 / q[0];
+    double pr;
+    if (P.dualenergy) {
+      if (kin / q[4] > P.de_thresh1) {
+        pr = q[5] * fast::pow_g1(P, q[0]);
+        q[4] = kin + pr * P.g1inv;
+      } else {
+        pr = P.g1 * (q[4] - kin);
+        q[5] = pr * fast::rpow_g1(P, q[0]);
+      }
+    } else {
+      pr = P.g1 * (q[4] - kin);
+    }
+    w[4] = pr;
+    return (pr <= 0.0) ? kErrNegPressure : kOk;
+  }
+  template <int LIMITER>
+  __device__ static __forceinline__ double slope3(double wl, double wc, double wr) {
+    return evh::slope3<LIMITER>(wl, wc, wr);
+  }
+  template <int D>
+  __device__ static __forceinline__ void predictor_dir(const KParams &P, const double w[5], double, const double s[5],
+                                                       double a[5]) {
+    evh::predictor_dir<D>(P, w, s, a);
+  }
+  template <int SOLVER, int DIR>
+  __device__ static __forceinline__ int riemann(const KParams &P, const double wl[5], const double wr[5], double f[6]) {
+    if (SOLVER == 1) return exact_bf::hll<DIR>(P, wl, wr, f);
+    return exact_bf::hllc<DIR>(P, wl, wr, f);
+  }
+  __device__ static __forceinline__ double cfl_cell(const KParams &P, const double w[5], double) {
+    return evh::cfl_cell(P, w);
+  }
+};
+
 // raw q at (gx, gy, gz), ghost-mapped (the index rules of `bound`)
 template <typename S>
 __device__ __forceinline__ void fetch_raw(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
@@ -125,12 +294,12 @@ __device__ __forceinline__ void fetch_raw(const KParams &P, const Geo &G, const 
 }
 
 // primitive of a fetched cell at column gx (for the reflecting wall)
-template <typename S, typename T>
+template <bool EXACT, typename S, typename T>
 __device__ __forceinline__ void prim_cell(const KParams &P, const Geo &G, const S raw[6], int gx, T qc[6], T wc[5],
                                           T &rinv) {
 #pragma unroll
   for (int m = 0; m < 6; m++) qc[m] = T(raw[m]);
-  Prim<T>::q2w(P, qc, wc, rinv);
+  Phys<EXACT>::q2w(P, qc, wc, rinv);
   if (P.boundtype == 3 && gx >= G.nx) {  // reflecting upper x (boundary.f90:57-64)
     wc[1] = -wc[1];
     wc[2] = -wc[2];
@@ -139,12 +308,12 @@ __device__ __forceinline__ void prim_cell(const KParams &P, const Geo &G, const 
 }
 
 // load q at (gx, gy, gz) (ghost-mapped) and recover w
-template <typename S, typename T>
+template <bool EXACT, typename S, typename T>
 __device__ __forceinline__ void load_cell(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
                                           int gz, T qc[6], T wc[5], T &rinv) {
   S raw[6];
   fetch_raw(P, G, q, gx, gy, gz, raw);
-  prim_cell(P, G, raw, gx, qc, wc, rinv);
+  prim_cell<EXACT>(P, G, raw, gx, qc, wc, rinv);
 }
 
 template <typename T, int EA>
@@ -165,7 +334,7 @@ struct Smem {
 // One tile `lin` (x fastest, then y, then z chunk) of the box whose q starts
 // at element qoff of qin_all/qout_all.  vmin_dst receives the CFL minimum of
 // the tile's new state; errors carry the box index above bit 40 of the cell.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT>
 __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__restrict__ qin_all,
                                       S *__restrict__ qout_all, long long qoff, int box, double dt, Status *st,
                                       unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm) {
@@ -231,8 +400,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   // next plane and pre2 the ring wave's second-ring cell, so that no wave
   // stalls on memory right after issuing a load
   S pre[6], pre2[6];
-  load_cell<S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
-  load_cell<S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
+  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
+  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
   fetch_raw(P, G, qin, gx, gy, z0, pre);
   fetch_raw(P, G, qin, x0 + r2x, y0 + r2y, z0 - 1, pre2);
 #pragma unroll
@@ -242,12 +411,12 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     const bool out_plane = k >= z0 && k < z1;
     // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
     if (owns_col) {
-      prim_cell(P, G, pre, gx, qn, wp, rp);
+      prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
       fetch_raw(P, G, qin, gx, gy, k + 2, pre);
     }
     if (ring && r2x > -3) {
       T rw[5], rr;
-      prim_cell(P, G, pre2, x0 + r2x, tq, rw, rr);
+      prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw, rr);
       store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw);
       fetch_raw(P, G, qin, x0 + r2x, y0 + r2y, k + 1, pre2);
     }
@@ -260,47 +429,51 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     if (owns_col) {
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-      sx[m] = fast::slope3<LIMITER>(W[m][me - 1], wc[m], W[m][me + 1]);
-      sy[m] = fast::slope3<LIMITER>(W[m][me - EW], wc[m], W[m][me + EW]);
-      sz[m] = fast::slope3<LIMITER>(wm[m], wc[m], wp[m]);
+      sx[m] = Phys<EXACT>::template slope3<LIMITER>(W[m][me - 1], wc[m], W[m][me + 1]);
+      sy[m] = Phys<EXACT>::template slope3<LIMITER>(W[m][me - EW], wc[m], W[m][me + EW]);
+      sz[m] = Phys<EXACT>::template slope3<LIMITER>(wm[m], wc[m], wp[m]);
     }
     if (P.secondorder) {
       T a1[5], a2[5], a3[5];
-      fast::predictor_dir<0>(P, wc, rc, sx, a1);
-      fast::predictor_dir<1>(P, wc, rc, sy, a2);
-      fast::predictor_dir<2>(P, wc, rc, sz, a3);
-      const T ex = dt2 * T(P.dxinv), ey = dt2 * T(P.dyinv), ez = dt2 * T(P.dzinv);
+      Phys<EXACT>::template predictor_dir<0>(P, wc, rc, sx, a1);
+      Phys<EXACT>::template predictor_dir<1>(P, wc, rc, sy, a2);
+      Phys<EXACT>::template predictor_dir<2>(P, wc, rc, sz, a3);
+      if (EXACT) {  // muscl.f90:106-107
+        const T ix = T(P.dxinv), iy = T(P.dyinv), iz = T(P.dzinv);
 #pragma unroll
-      for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey + a3[m] * ez;
+        for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * ix + a2[m] * iy + a3[m] * iz);
+      } else {
+        const T ex = dt2 * T(P.dxinv), ey = dt2 * T(P.dyinv), ez = dt2 * T(P.dzinv);
+#pragma unroll
+        for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey + a3[m] * ez;
+      }
     } else {
 #pragma unroll
       for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }
     }
-    // face states (muscl.f90:207-210): w + evo + 0.5 s on the high side,
-    // w + evo - 0.5 s on the low side
-#pragma unroll
-    for (int m = 0; m < 5; m++) ev[m] = wc[m] + ev[m];  // ev now holds w + evo
+    // face states (muscl.f90:207-210) in the reference's order:
+    // (w + 0.5 s) + evo on the high side, (w - 0.5 s) + evo on the low side
     if (!ring) {
       const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        XB[5 + m][fxl] = ev[m] - half * sx[m];       // right state of the left face
-        XB[m][fxl + 1] = ev[m] + half * sx[m];       // left state of the right face
-        YB[5 + m][fyb] = ev[m] - half * sy[m];
-        YB[m][fyb + TX] = ev[m] + half * sy[m];
+        XB[5 + m][fxl] = (wc[m] - half * sx[m]) + ev[m];       // right state of the left face
+        XB[m][fxl + 1] = (wc[m] + half * sx[m]) + ev[m];       // left state of the right face
+        YB[5 + m][fyb] = (wc[m] - half * sy[m]) + ev[m];
+        YB[m][fyb + TX] = (wc[m] + half * sy[m]) + ev[m];
       }
     } else if (side == 0) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = ev[m] + half * sx[m];
+      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = (wc[m] + half * sx[m]) + ev[m];
     } else if (side == 1) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = ev[m] - half * sx[m];
+      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = (wc[m] - half * sx[m]) + ev[m];
     } else if (side == 2) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) YB[m][cx] = ev[m] + half * sy[m];
+      for (int m = 0; m < 5; m++) YB[m][cx] = (wc[m] + half * sy[m]) + ev[m];
     } else {
 #pragma unroll
-      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = ev[m] - half * sy[m];
+      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = (wc[m] - half * sy[m]) + ev[m];
     }
     // z face k-1/2 (interior lanes) and the end of plane k-1; the
     // scheduling barrier keeps this solve apart from the x/y solves
@@ -309,14 +482,14 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       T zl[5], zr[5];
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        zl[m] = ev[m] - half * sz[m];
-        zr[m] = ev[m] + half * sz[m];
+        zl[m] = (wc[m] - half * sz[m]) + ev[m];
+        zr[m] = (wc[m] + half * sz[m]) + ev[m];
       }
       if (k >= z0) {
         T zrp[5], fz[6];
 #pragma unroll
         for (int m = 0; m < 5; m++) zrp[m] = CR[12 + m][tid];
-        const int e = fast::riemann<SOLVER, 3>(P, zrp, zl, fz);
+        const int e = Phys<EXACT>::template riemann<SOLVER, 3>(P, zrp, zl, fz);
         if (e && inside && !err) { err = e; err_k = k; }
         if (k - 1 >= z0 && inside) {
           T o[6];
@@ -328,9 +501,9 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
           T on[6], wn[5], rn;
 #pragma unroll
           for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
-          const int ep = Prim<T>::q2w(P, on, wn, rn);
+          const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
           if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
-          vmin = dmin(vmin, fast::cfl_cell(P, wn, rn));
+          vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
         }
 #pragma unroll
         for (int m = 0; m < 6; m++) CR[6 + m][tid] = fz[m];
@@ -355,8 +528,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
           yl[m] = YB[m][fyi];
           yr[m] = YB[5 + m][fyi];
         }
-        const int ex = fast::riemann<SOLVER, 1>(P, xl, xr, fx);
-        const int ey = fast::riemann<SOLVER, 2>(P, yl, yr, fy);
+        const int ex = Phys<EXACT>::template riemann<SOLVER, 1>(P, xl, xr, fx);
+        const int ey = Phys<EXACT>::template riemann<SOLVER, 2>(P, yl, yr, fy);
         // faces of cells outside the box (partial tiles) do not count
         const int e = (gy < G.ny && gx <= G.nx && ex) ? ex : ((gx < G.nx && gy <= G.ny) ? ey : 0);
         const bool take = e && !err;
@@ -376,7 +549,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
           T wl[5], wr[5], f[6];
 #pragma unroll
           for (int m = 0; m < 5; m++) { wl[m] = XB[m][fxi]; wr[m] = XB[5 + m][fxi]; }
-          const int e = fast::riemann<SOLVER, 1>(P, wl, wr, f);
+          const int e = Phys<EXACT>::template riemann<SOLVER, 1>(P, wl, wr, f);
           if (e && !err && x0 + TX <= G.nx && y0 + lane < G.ny) { err = e; err_k = k; }
 #pragma unroll
           for (int m = 0; m < 6; m++) XB[m][fxi] = f[m];
@@ -385,7 +558,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
           T wl[5], wr[5], f[6];
 #pragma unroll
           for (int m = 0; m < 5; m++) { wl[m] = YB[m][fyi]; wr[m] = YB[5 + m][fyi]; }
-          const int e = fast::riemann<SOLVER, 2>(P, wl, wr, f);
+          const int e = Phys<EXACT>::template riemann<SOLVER, 2>(P, wl, wr, f);
           if (e && !err && x0 + lane - TY < G.nx && y0 + TY <= G.ny) { err = e; err_k = k; }
 #pragma unroll
           for (int m = 0; m < 6; m++) YB[m][fyi] = f[m];

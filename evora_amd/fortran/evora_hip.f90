@@ -21,7 +21,7 @@ module evora_hip
   integer(c_int), parameter :: EVH_ERR_HIP = 9
   integer(c_int), parameter :: EVH_ERR_STATE = 10
   integer(c_int), parameter :: EVH_FP64 = 0, EVH_FP32_MIXED = 1
-  integer(c_int), parameter :: EVH_MODE_FUSED = 0, EVH_MODE_STAGED = 1
+  integer(c_int), parameter :: EVH_MODE_FUSED = 0, EVH_MODE_STAGED = 1, EVH_MODE_FUSED_EXACT = 2
 
   type, bind(c) :: evh_config
      integer(c_int) :: nx, ny, nz

@@ -13,6 +13,9 @@
 !
 ! Further environment switches:
 !   EVORA_PRECISION=fp64|mixed   FP64 (default) or the FP32-mixed form
+!   EVORA_EXACT=1                FP64 with the reference's own arithmetic in
+!                                the single-pass kernel (bit-identical to the
+!                                CPU path except where a power enters)
 !   EVORA_MAXSTEPS=N             after N steps, set tend = t in hydrostep so
 !                                that updatetime (timestep.f90:163-166) ends
 !                                the run with a dt = 0 step (both backends)
@@ -39,6 +42,7 @@ module hydro_dispatch_state
   integer :: maxsteps = -1, nsteps_seen = 0
   integer :: k2s = 1, k3s = 1        ! shadows of main's evolution/snapshot counters
   integer(c_int) :: precision = EVH_FP64
+  logical :: exact = .false.
 
 contains
 
@@ -55,6 +59,8 @@ contains
     if (st .eq. 0 .and. l .gt. 0) then
        if (v(1:l) .eq. 'mixed') precision = EVH_FP32_MIXED
     endif
+    call get_environment_variable('EVORA_EXACT', v, l, st)
+    if (st .eq. 0 .and. l .gt. 0) exact = v(1:l) .eq. '1'
     call get_environment_variable('EVORA_MAXSTEPS', v, l, st)
     if (st .eq. 0 .and. l .gt. 0) read (v(1:l), *) maxsteps
   end subroutine decide_backend
@@ -135,6 +141,9 @@ contains
     cfg%volume = volume
     cfg%mach_fp64 = 5.0d0
     call hip_check(evh_create(cfg, ctx))
+    ! 3-D grids start in the fused form; the exact form on request (FP64)
+    if (exact .and. precision .eq. EVH_FP64 .and. .not. onedimension .and. .not. twodimension) &
+         call hip_check(evh_set_mode(ctx, EVH_MODE_FUSED_EXACT))
   end subroutine hip_create
 
 end module hydro_dispatch_state

@@ -23,7 +23,7 @@ from .params import Derived, HydroParams
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libevora_hip.so")
 
 FP64, FP32_MIXED = 0, 1
-MODE_FUSED, MODE_STAGED = 0, 1
+MODE_FUSED, MODE_STAGED, MODE_FUSED_EXACT = 0, 1, 2
 
 ERR_NAMES = {
     1: "negative pressure", 2: "riemann unphysical pressure", 3: "hll", 4: "hllc", 5: "dt",

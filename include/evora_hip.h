@@ -37,7 +37,7 @@ enum {
 };
 
 enum { EVH_FP64 = 0, EVH_FP32_MIXED = 1 };
-enum { EVH_MODE_FUSED = 0, EVH_MODE_STAGED = 1 };
+enum { EVH_MODE_FUSED = 0, EVH_MODE_STAGED = 1, EVH_MODE_FUSED_EXACT = 2 };
 
 /* Constants of module `global` the step reads (global.f90:44-86), as derived
  * by compparameter (parameter.f90:164-265).  The host passes them; the
@@ -63,9 +63,14 @@ typedef struct evh_ctx evh_ctx;
 int evh_create(const evh_config *cfg, evh_ctx **out);
 int evh_destroy(evh_ctx *h);
 
-/* Select the kernel structure: EVH_MODE_FUSED (one HBM pass per step, the
- * default for 3-D grids) or EVH_MODE_STAGED (one kernel per reference
- * routine, with w, evo and s materialised in HBM; the path for 1-D/2-D). */
+/* Select the kernel structure: EVH_MODE_FUSED (one HBM pass per step with
+ * refined hardware reciprocals and roots, results within a few ulp of the
+ * reference's), EVH_MODE_FUSED_EXACT (the same single pass with the
+ * reference's IEEE arithmetic and operation order, FP64 only: bit-identical
+ * to the reference except where the entropy takes a power), or
+ * EVH_MODE_STAGED (one kernel per reference routine, with w, evo and s
+ * materialised in HBM; the path for 1-D/2-D).  3-D grids start in
+ * EVH_MODE_FUSED. */
 int evh_set_mode(evh_ctx *h, int mode);
 
 /* Host q (reference layout) <-> device state. */

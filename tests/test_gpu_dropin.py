@@ -66,3 +66,18 @@ def test_dropin_3d_fused_path_with_maxsteps(precision):
     tol = 1e-10 if precision == "fp64" else 1e-5
     for c in range(6):
         assert rel_l1(qh[..., c], qc[..., c]) <= tol
+
+
+def test_dropin_exact_matches_reference_bitwise():
+    """EVORA_EXACT=1: the GPU runs the fused step with the reference's own
+    arithmetic; the run (t and dt of every step, and q) equals the compiled
+    reference's bit for bit except the entropy, where a power enters."""
+    p = HydroParams(nx=30, ny=26, nz=20, solver=2, limiter=1, boundtype=3, cfl=0.4, atomic=True, tend=10.0)
+    q0 = ics.random_smooth(p, seed=22, amp=0.1)
+    env = {"EVORA_MAXSTEPS": "8", "EVORA_EXACT": "1"}
+    qc, sc = run_dropin(p, q0, 1000, "cpu", env)
+    qh, sh = run_dropin(p, q0, 1000, "hip", env)
+    assert sc == sh
+    for c in range(5):
+        assert np.array_equal(qh[..., c], qc[..., c]), c
+    assert rel_linf(qh[..., 5], qc[..., 5]) <= 1e-13
