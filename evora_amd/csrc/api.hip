@@ -238,10 +238,11 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
   h->qelem = mixed(h) ? sizeof(float) : sizeof(double);
   h->mode = fused_supported(*cfg) ? EVH_MODE_FUSED : EVH_MODE_STAGED;
   // launch shape measured fastest on MI355X for the 512^3 blast (16x12 tiles,
-  // 64-plane chunks; FP64 at 2 waves/SIMD, FP32 at 3)
+  // 64-plane chunks, the higher waves-per-SIMD budget: FP64 runs at 2
+  // waves/SIMD, FP32 at 4)
   h->tile_ty = 12;
   h->zchunk = 64;
-  h->wpe_hi = !mixed(h);
+  h->wpe_hi = 1;
 
   auto bail = [&](hipError_t e) {
     int r = hip_fail(h, e);

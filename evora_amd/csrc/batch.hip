@@ -457,7 +457,11 @@ int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
   b->cfgs.assign(cfgs, cfgs + nbox);
   b->precision = c0.precision;
   b->qelem = b->precision == EVH_FP32_MIXED ? 4 : 8;
-  b->wpe_hi = b->precision == EVH_FP32_MIXED ? 0 : 1;
+  // default launch shape (the autotuner may pick another): 16x12 tiles, the
+  // higher waves-per-SIMD budget, as many resident blocks per CU as that
+  // occupancy allows (FP32: 4 waves/SIMD, FP64: 2)
+  b->wpe_hi = 1;
+  b->blocks_per_cu = b->precision == EVH_FP32_MIXED ? 4 : 2;
   long long off = 0;
   for (int i = 0; i < nbox; i++) {
     b->kps.push_back(make_kparams(cfgs[i]));

@@ -17,6 +17,11 @@ namespace fast {
 // fastmath.h when g1 is close to 2/3 and rho is inside 2^(+-200), the
 // general exp(y ln x) otherwise; single precision uses the hardware
 // exp2/log2.
+// the general power, out of line: the cube-root form covers the hot cases,
+// and inlined here the series constants would occupy registers for the
+// whole of a kernel's loop
+__device__ __attribute__((noinline)) double pow_general(double x, double y) { return fm::pow_pos(x, y); }
+
 __device__ __forceinline__ double rpow_g1(const KParams &P, double x) {
   const int e = __builtin_amdgcn_frexp_exp(x);
   if (P.g1_third && e > -200 && e < 200) {
@@ -25,7 +30,7 @@ __device__ __forceinline__ double rpow_g1(const KParams &P, double x) {
     const double z = y * y;
     return __builtin_fma(z, fm::expm1_tiny(c), z);
   }
-  return fm::pow_pos(x, -P.g1);
+  return pow_general(x, -P.g1);
 }
 __device__ __forceinline__ double pow_g1(const KParams &P, double x) {
   const int e = __builtin_amdgcn_frexp_exp(x);
@@ -35,7 +40,7 @@ __device__ __forceinline__ double pow_g1(const KParams &P, double x) {
     const double v = x * y;
     return __builtin_fma(v, fm::expm1_tiny(-c), v);
   }
-  return fm::pow_pos(x, P.g1);
+  return pow_general(x, P.g1);
 }
 __device__ __forceinline__ float rpow_g1(const KParams &P, float x) { return fm::pow_pos(x, -float(P.g1)); }
 __device__ __forceinline__ float pow_g1(const KParams &P, float x) { return fm::pow_pos(x, float(P.g1)); }
