@@ -11,6 +11,11 @@
 !   EVORA_BACKEND=cpu (default)  calls the reference routines unchanged;
 !   EVORA_BACKEND=hip            runs the step on the GPU through libevora_hip.
 !
+! It also defines evolution (snapshot.f90:159-193), the driver's periodic
+! report of total mass, energy and entropy: once the state lives on the
+! device the three sums are taken there (evh_sums, FP64 compensated), so an
+! evolution step moves 24 bytes instead of the whole of q.
+!
 ! Further environment switches:
 !   EVORA_PRECISION=fp64|mixed   FP64 (default) or the FP32-mixed form
 !   EVORA_EXACT=1                FP64 with the reference's own arithmetic in
@@ -22,8 +27,8 @@
 !
 ! On the GPU the conserved state stays on the device.  It is uploaded at the
 ! first primitive call and downloaded after sweep only when the driver is
-! about to read q: when evolution or snapshot will run (the tsnap tests of
-! main.f90:181-195, mirrored with shadow counters) and on the last step
+! about to read q: when snapshot will run (the tsnap test of
+! main.f90:192-195, mirrored with a shadow counter) and on the last step
 ! (tend - t <= dt, the test of timestep.f90:163).  w, evo and s are never
 ! filled on the host; pure hydro never reads them there.  Physics that
 ! update q on the host between steps (gravity, chemistry, floors,
@@ -40,7 +45,7 @@ module hydro_dispatch_state
   logical :: on_device = .false.
   logical :: last_step = .false.
   integer :: maxsteps = -1, nsteps_seen = 0
-  integer :: k2s = 1, k3s = 1        ! shadows of main's evolution/snapshot counters
+  integer :: k3s = 1                 ! shadow of main's snapshot counter
   integer(c_int) :: precision = EVH_FP64
   logical :: exact = .false.
 
@@ -232,16 +237,47 @@ subroutine sweep(w, q, evo, s)
      return
   endif
   call hip_check(evh_sweep(ctx, dt))
-  ! main.f90:181-195 after this step (t already advanced by updatetime)
+  ! main.f90:181-195 after this step (t already advanced by updatetime);
+  ! evolution takes its sums on the device and needs no q
   need_q = last_step
   tsnap_next = (t - tstart) / (tend - tstart) * nsnap
-  if (tsnap_next .ge. dble(k2s)) then
-     need_q = .true.
-     k2s = k2s + 1
-  endif
   if ((cut .or. slice .or. bov) .and. tsnap_next .ge. dble(k3s)) then
      need_q = .true.
      k3s = k3s + 1
   endif
   if (need_q) call hip_check(evh_download_q(ctx, q))
 end subroutine sweep
+
+! snapshot.f90:159-193.  Before the state is on the device (the call of
+! main.f90:100) and on the CPU backend this is the reference routine; on the
+! device the sums come from evh_sums and the line written is the
+! reference's (same file, header and format).  The GPU backend runs on one
+! rank, so the parallel reductions and the barrier of the reference are not
+! needed.
+subroutine evolution(k2, q)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  integer, intent(inout) :: k2
+  real(8), intent(in), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+  real(c_double) :: sums(3)
+  real(8) :: totalmass, totalenergy, totalentropy
+  integer :: u
+
+  call decide_backend()
+  if (backend .eq. 0 .or. .not. on_device) then
+     call evcpu_evolution(k2, q)
+     return
+  endif
+  call hip_check(evh_sums(ctx, sums))
+  totalmass = sums(1) * volume
+  totalenergy = sums(2) * volume
+  totalentropy = sums(3) * volume
+  if (master) then
+     open (newunit=u, file="data/evo", access="sequential", status="unknown", position="append")
+     if (k2 .eq. 0) write (u,'(A,A15,23A16)') "#","t","dt","a","z","totalmass","totalenergy","totalentropy"
+     write (u,'(24E16.6)') t, dt, cosmo, redshift, totalmass, totalenergy, totalentropy
+     close (u)
+  endif
+  k2 = k2 + 1
+end subroutine evolution

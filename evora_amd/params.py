@@ -43,6 +43,7 @@ class HydroParams:
     de_thresh: float = 0.0
     tstart: float = 0.0
     tend: float = 1.0
+    nsnap: int = 100  # output count over [tstart, tend] (parameter.f90:55)
 
     # ---- parameter.f90:164-265 ----
     def derive(self) -> "Derived":
@@ -94,7 +95,7 @@ class HydroParams:
             f" zmin = {r(self.zmin)}, zmax = {r(self.zmax)},\n"
             f" boundtype = {self.boundtype},\n"
             f" dualenergy = {b(self.dualenergy)}, de_thresh = {r(self.de_thresh)},\n"
-            f" tstart = {r(self.tstart)}, tend = {r(self.tend)}\n"
+            f" tstart = {r(self.tstart)}, tend = {r(self.tend)}, nsnap = {self.nsnap}\n"
             "/\n"
         )
 

@@ -29,9 +29,9 @@ program evref
   real(8), allocatable :: q(:,:,:,:), w(:,:,:,:), evo(:,:,:,:), s(:,:,:,:,:)
   real(8), allocatable :: rec(:)
   real(8) :: wl(nw), wr(nw), wc(nw), f(nf), sl(nw)
-  integer :: nsteps, k, n, i, u, fsize, dir, k4
+  integer :: nsteps, k, n, i, u, fsize, dir, k4, k2, l, st
   integer(8) :: c0, c1, crate
-  logical :: go, dump
+  logical :: go, dump, evolve
 
   call get_command_argument(2, mode)
   call get_command_argument(3, fin)
@@ -103,6 +103,15 @@ program evref
   go = .true.
   dump = .false.
   k4 = 1
+  ! EVREF_EVOLUTION=1: after each step, the periodic totals of
+  ! main.f90:184-191 (not comoving).  The counter starts at 1, as after the
+  ! call of main.f90:100 before the loop, which this harness does not make:
+  ! the reference evolution ends in the MPI barrier of parallel.f90, which is
+  ! not linked here, so the option is for the GPU drop-in (whose evolution
+  ! takes the sums on the device) only.
+  call get_environment_variable('EVREF_EVOLUTION', arg, l, st)
+  evolve = st .eq. 0 .and. l .gt. 0
+  k2 = 1
   if (trim(mode) .eq. 'stage') nsteps = 1
   call system_clock(c0, crate)
   do k = 1, nsteps
@@ -121,6 +130,10 @@ program evref
      endif
      call sweep(w, q, evo, s)
      write (*, '(A,I6,2ES25.16)') 'STEP ', k, t, dt
+     if (evolve) then
+        tsnap = (t - tstart) / (tend - tstart) * nsnap
+        if (tsnap .ge. dble(k2)) call evolution(k2, q)
+     endif
   enddo
   call system_clock(c1)
   write (*, '(A,ES16.8)') 'LOOP_SECONDS ', dble(c1 - c0) / dble(crate)

@@ -20,11 +20,13 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(os.path.dirname(O.REF_PATH), "evref_hip")
 
 
-def run_dropin(params, q0, nsteps, backend, extra_env=None):
+def run_dropin(params, q0, nsteps, backend, extra_env=None, evo=None):
+    """Run the harness; with evo=[] the lines of data/evo are appended to it."""
     env = dict(os.environ, EVORA_BACKEND=backend)
     if extra_env:
         env.update(extra_env)
     with tempfile.TemporaryDirectory() as td:
+        os.makedirs(os.path.join(td, "data"))
         para = os.path.join(td, "para")
         with open(para, "w") as fh:
             fh.write(params.to_namelist())
@@ -34,6 +36,9 @@ def run_dropin(params, q0, nsteps, backend, extra_env=None):
                            text=True, timeout=300)
         assert "ERROR" not in r.stdout, r.stdout[-2000:]
         steps = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("STEP")]
+        if evo is not None and os.path.exists(os.path.join(td, "data", "evo")):
+            with open(os.path.join(td, "data", "evo")) as fh:
+                evo.extend(fh.read().splitlines())
         return np.fromfile(fout).reshape(q0.shape), steps
 
 
@@ -81,3 +86,34 @@ def test_dropin_exact_matches_reference_bitwise():
     for c in range(5):
         assert np.array_equal(qh[..., c], qc[..., c]), c
     assert rel_linf(qh[..., 5], qc[..., 5]) <= 1e-13
+
+
+def test_dropin_evolution_on_device():
+    """With the state on the GPU, evolution (snapshot.f90:159-193) takes the
+    totals on the device instead of downloading q.  The lines of data/evo
+    follow main.f90's rule (after a step, when (t - tstart)/(tend - tstart)
+    * nsnap >= k2), carry the step's t, keep the periodic box's mass, and
+    the last one equals the totals of the final state in the reference's
+    E16.6 format."""
+    p = HydroParams(nx=30, ny=26, nz=20, solver=2, limiter=1, boundtype=2, cfl=0.4, atomic=True,
+                    tend=0.1, nsnap=5)
+    q0 = ics.random_smooth(p, seed=23, amp=0.1)
+    lines = []
+    q1, steps = run_dropin(p, q0, 100000, "hip", {"EVREF_EVOLUTION": "1"}, evo=lines)
+    rows = [[float(x) for x in ln.split()] for ln in lines if not ln.startswith("#")]
+    ts = [float(st[2]) for st in steps]
+    expect, k2 = [], 1
+    for t in ts:
+        if (t - p.tstart) / (p.tend - p.tstart) * p.nsnap >= k2:
+            expect.append(t)
+            k2 += 1
+    assert len(rows) == len(expect) >= p.nsnap
+    # E16.6 prints six significant digits: a value is exact to 5e-6 of itself
+    for row, t in zip(rows, expect):
+        assert row[0] == pytest.approx(t, rel=1e-5)
+    vol = p.derive().volume
+    totals = [q1[..., 0].sum() * vol, q1[..., 4].sum() * vol, q1[..., 5].sum() * vol]
+    for got, want in zip(rows[-1][4:7], totals):
+        assert got == pytest.approx(want, rel=1e-5)
+    for row in rows:
+        assert row[4] == pytest.approx(rows[0][4], rel=1e-5)
