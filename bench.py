@@ -90,7 +90,7 @@ def drift(s_end, s_0):
     return dict(mass=float((s_end[0] - s_0[0]) / s_0[0]), energy=float((s_end[1] - s_0[1]) / s_0[1]))
 
 
-def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False, exact=False):
+def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False, exact=False, time_xfer=False):
     """Advance q0 warmup + steps steps with the fused path (exact: the FP64
     fused step with the reference's own arithmetic); the `steps` steps are
     timed with the state resident in HBM."""
@@ -100,7 +100,9 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
     h = hydro.Hydro(d, precision=prec, mode=hydro.MODE_FUSED_EXACT if exact else hydro.MODE_FUSED)
     try:
         log(f"{precision_name}: uploading {q0.nbytes / 1e9:.2f} GB; device bytes {h.device_bytes() / 1e9:.2f} GB")
+        t_up = time.perf_counter()
         h.upload_q(q0)
+        t_up = time.perf_counter() - t_up
         sums0 = h.sums()
         t = 0.0
         for _ in range(warmup):
@@ -114,7 +116,9 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
         wall = time.perf_counter() - t0
         kms, launches = h.kernel_stats()
         sums = h.sums()
-        q = h.download_q() if keep_q else None
+        t_dn = time.perf_counter()
+        q = h.download_q() if keep_q or time_xfer else None
+        t_dn = time.perf_counter() - t_dn
     finally:
         h.close()
     res = dict(
@@ -127,6 +131,9 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
         sums=sums,
         q=q,
     )
+    if time_xfer:  # whole-state host transfers (reference layout, pageable host array)
+        res["upload_GBs"] = q0.nbytes / t_up / 1e9
+        res["download_GBs"] = q0.nbytes / t_dn / 1e9
     log(f"{precision_name}: {res['value'] / 1e9:.2f} G cell-updates/s, {res['ms_per_step']:.3f} ms/step, "
         f"kernel {res['kernel_ms']:.3f} ms")
     return res
@@ -349,10 +356,11 @@ def main():
     log(f"building {args.n}^3 blast initial condition")
     q0 = ics.blast(p)
     acc = not args.no_accuracy and not args.fp64_only
-    fp64 = run_gpu(d, q0, args.steps, args.warmup, "fp64", keep_q=acc)
+    fp64 = run_gpu(d, q0, args.steps, args.warmup, "fp64", keep_q=acc, time_xfer=True)
     achieved = BYTES_PER_CELL["fp64"] * args.n ** 3 / (fp64["kernel_ms"] * 1e-3) / 1e9
     forms = {"fp64": dict(value=fp64["value"], ms_per_step=fp64["ms_per_step"], kernel_ms=fp64["kernel_ms"],
-                          hbm_frac=achieved / HBM_PEAK_GBS)}
+                          hbm_frac=achieved / HBM_PEAK_GBS, upload_GBs=fp64["upload_GBs"],
+                          download_GBs=fp64["download_GBs"])}
     ex = run_gpu(d, q0, args.steps, args.warmup, "fp64", exact=True)
     forms["fp64_exact"] = dict(value=ex["value"], ms_per_step=ex["ms_per_step"], kernel_ms=ex["kernel_ms"],
                                hbm_frac=BYTES_PER_CELL["fp64"] * args.n ** 3 / (ex["kernel_ms"] * 1e-3) / 1e9

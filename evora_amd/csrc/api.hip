@@ -261,16 +261,11 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
     if ((e = hipMalloc(&h->qbuf[b], h->qelem * 6 * n)) != hipSuccess) return bail(e);
     h->dev_bytes += (long long)h->qelem * 6 * n;
   }
-  const long long plane = 6LL * cfg->nx * cfg->ny;
-  long long planes = (32LL << 20) / plane;  // ~256 MB of staging
-  if (planes < 1) planes = 1;
-  if (planes > cfg->nz) planes = cfg->nz;
-  h->stage_elems = (size_t)(planes * plane);
-  if ((e = hipMalloc(&h->stage, sizeof(double) * h->stage_elems)) != hipSuccess) return bail(e);
+  if ((e = h->xfer.init(n)) != hipSuccess) return bail(e);
   if ((e = hipMalloc(&h->st, sizeof(Status))) != hipSuccess) return bail(e);
   if ((e = hipHostMalloc(&h->st_host, sizeof(Status))) != hipSuccess) return bail(e);
   if ((e = hipMalloc(&h->partials, sizeof(double) * 3 * kSumBlocks)) != hipSuccess) return bail(e);
-  h->dev_bytes += (long long)sizeof(double) * h->stage_elems + sizeof(Status) + sizeof(double) * 3 * kSumBlocks;
+  h->dev_bytes += h->xfer.device_bytes() + (long long)sizeof(Status) + (long long)sizeof(double) * 3 * kSumBlocks;
   hipLaunchKernelGGL(k_clear_code, dim3(1), dim3(64), 0, h->stream, h->st);
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(e);
   *out = h;
@@ -284,7 +279,7 @@ int evh_destroy(evh_ctx *h) {
   if (h->w) hipFree(h->w);
   if (h->evo) hipFree(h->evo);
   if (h->s) hipFree(h->s);
-  if (h->stage) hipFree(h->stage);
+  h->xfer.release();
   if (h->st) hipFree(h->st);
   if (h->st_host) hipHostFree(h->st_host);
   if (h->partials) hipFree(h->partials);
@@ -309,19 +304,17 @@ int evh_set_mode(evh_ctx *h, int mode) {
 int evh_upload_q(evh_ctx *h, const double *q) {
   if (!h || !q) return EVH_ERR_STATE;
   const long long n = h->grid.n();
-  const long long chunk = (long long)h->stage_elems / 6;
-  for (long long base = 0; base < n; base += chunk) {
-    const long long cells = (n - base < chunk) ? n - base : chunk;
-    HIP_TRY(hipMemcpyAsync(h->stage, q + 6 * base, sizeof(double) * 6 * cells, hipMemcpyHostToDevice, h->stream));
-    if (mixed(h))
-      hipLaunchKernelGGL(k_aos_to_soa<float>, dim3(grid_of(cells)), dim3(256), 0, h->stream, h->stage,
-                         (float *)h->qbuf[h->cur], cells, base, n);
+  void *dst = h->qbuf[h->cur];
+  const bool f32 = mixed(h);
+  hipStream_t s = h->stream;
+  HIP_TRY(h->xfer.upload(q, n, s, [=](const double *stage, long long cells, long long base) {
+    if (f32)
+      hipLaunchKernelGGL(k_aos_to_soa<float>, dim3(grid_of(cells)), dim3(256), 0, s, stage, (float *)dst, cells, base, n);
     else
-      hipLaunchKernelGGL(k_aos_to_soa<double>, dim3(grid_of(cells)), dim3(256), 0, h->stream, h->stage,
-                         (double *)h->qbuf[h->cur], cells, base, n);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(h->stream));
-  }
+      hipLaunchKernelGGL(k_aos_to_soa<double>, dim3(grid_of(cells)), dim3(256), 0, s, stage, (double *)dst, cells,
+                         base, n);
+    return hipGetLastError();
+  }));
   h->v_valid = false;
   h->w_valid = false;
   return EVH_OK;
@@ -332,19 +325,18 @@ int evh_download_q(evh_ctx *h, double *q) {
   int r = check_status(h);
   if (r) return r;
   const long long n = h->grid.n();
-  const long long chunk = (long long)h->stage_elems / 6;
-  for (long long base = 0; base < n; base += chunk) {
-    const long long cells = (n - base < chunk) ? n - base : chunk;
-    if (mixed(h))
-      hipLaunchKernelGGL(k_soa_to_aos<float>, dim3(grid_of(cells)), dim3(256), 0, h->stream,
-                         (const float *)h->qbuf[h->cur], h->stage, cells, base, n);
+  const void *src = h->qbuf[h->cur];
+  const bool f32 = mixed(h);
+  hipStream_t s = h->stream;
+  HIP_TRY(h->xfer.download(q, n, s, [=](double *stage, long long cells, long long base) {
+    if (f32)
+      hipLaunchKernelGGL(k_soa_to_aos<float>, dim3(grid_of(cells)), dim3(256), 0, s, (const float *)src, stage, cells,
+                         base, n);
     else
-      hipLaunchKernelGGL(k_soa_to_aos<double>, dim3(grid_of(cells)), dim3(256), 0, h->stream,
-                         (const double *)h->qbuf[h->cur], h->stage, cells, base, n);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(q + 6 * base, h->stage, sizeof(double) * 6 * cells, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-  }
+      hipLaunchKernelGGL(k_soa_to_aos<double>, dim3(grid_of(cells)), dim3(256), 0, s, (const double *)src, stage,
+                         cells, base, n);
+    return hipGetLastError();
+  }));
   return EVH_OK;
 }
 

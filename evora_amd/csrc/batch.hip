@@ -227,8 +227,7 @@ struct evh_batch {
   void *qbuf[2] = {nullptr, nullptr};
   int cur = 0;
   void *snap = nullptr;  // autotuner snapshot of the state
-  double *stage = nullptr;
-  size_t stage_elems = 0;
+  evh::Xfer xfer;  // host transfers (pinned, double-buffered chunks)
   BoxDev *boxes_d = nullptr;
   int nitems = 0;
   Queues *qs_d = nullptr;
@@ -486,8 +485,7 @@ int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
     if (hipMalloc(&b->qbuf[i], b->qelem * 6 * b->total) != hipSuccess) return bail(EVH_ERR_HIP);
   long long maxn = 0;
   for (auto n : b->ncell) maxn = n > maxn ? n : maxn;
-  b->stage_elems = (size_t)(6 * (maxn < (8LL << 20) ? maxn : (8LL << 20)));
-  if (hipMalloc(&b->stage, 8 * b->stage_elems) != hipSuccess) return bail(EVH_ERR_HIP);
+  if (b->xfer.init(maxn) != hipSuccess) return bail(EVH_ERR_HIP);
   if (hipMalloc(&b->boxes_d, sizeof(BoxDev) * nbox) != hipSuccess) return bail(EVH_ERR_HIP);
   if (hipMalloc(&b->qs_d, sizeof(Queues)) != hipSuccess) return bail(EVH_ERR_HIP);
   BoxState &s = b->bs;
@@ -526,7 +524,7 @@ int evh_batch_destroy(evh_batch *b) {
   for (auto p : b->qbuf)
     if (p) hipFree(p);
   if (b->snap) hipFree(b->snap);
-  if (b->stage) hipFree(b->stage);
+  b->xfer.release();
   if (b->boxes_d) hipFree(b->boxes_d);
   if (b->qs_d) hipFree(b->qs_d);
   void *arrs[] = {b->bs.t, b->bs.tend, b->bs.dt, b->bs.cfl, b->bs.go, b->bs.active, b->bs.steps, b->bs.err_step,
@@ -542,38 +540,40 @@ int evh_batch_destroy(evh_batch *b) {
 
 int evh_batch_upload_q(evh_batch *b, int box, const double *q) {
   if (!b || !q || box < 0 || box >= b->nbox) return EVH_ERR_STATE;
-  const long long n = b->ncell[box], chunk = (long long)b->stage_elems / 6;
-  for (long long base = 0; base < n; base += chunk) {
-    const long long cells = n - base < chunk ? n - base : chunk;
-    BHIP(hipMemcpyAsync(b->stage, q + 6 * base, 8 * 6 * cells, hipMemcpyHostToDevice, b->stream));
-    if (b->precision == EVH_FP32_MIXED)
-      hipLaunchKernelGGL(k_b_aos_to_soa<float>, dim3(gridsz(cells)), dim3(256), 0, b->stream, b->stage,
-                         (float *)b->qbuf[b->cur] + 6 * b->qoff[box], cells, base, n);
+  const long long n = b->ncell[box];
+  const bool f32 = b->precision == EVH_FP32_MIXED;
+  void *dst = f32 ? (void *)((float *)b->qbuf[b->cur] + 6 * b->qoff[box])
+                  : (void *)((double *)b->qbuf[b->cur] + 6 * b->qoff[box]);
+  hipStream_t s = b->stream;
+  BHIP(b->xfer.upload(q, n, s, [=](const double *stage, long long cells, long long base) {
+    if (f32)
+      hipLaunchKernelGGL(k_b_aos_to_soa<float>, dim3(gridsz(cells)), dim3(256), 0, s, stage, (float *)dst, cells, base,
+                         n);
     else
-      hipLaunchKernelGGL(k_b_aos_to_soa<double>, dim3(gridsz(cells)), dim3(256), 0, b->stream, b->stage,
-                         (double *)b->qbuf[b->cur] + 6 * b->qoff[box], cells, base, n);
-    BHIP(hipGetLastError());
-    BHIP(hipStreamSynchronize(b->stream));
-  }
+      hipLaunchKernelGGL(k_b_aos_to_soa<double>, dim3(gridsz(cells)), dim3(256), 0, s, stage, (double *)dst, cells,
+                         base, n);
+    return hipGetLastError();
+  }));
   b->cfl_needed[box] = 1;
   return EVH_OK;
 }
 
 int evh_batch_download_q(evh_batch *b, int box, double *q) {
   if (!b || !q || box < 0 || box >= b->nbox) return EVH_ERR_STATE;
-  const long long n = b->ncell[box], chunk = (long long)b->stage_elems / 6;
-  for (long long base = 0; base < n; base += chunk) {
-    const long long cells = n - base < chunk ? n - base : chunk;
-    if (b->precision == EVH_FP32_MIXED)
-      hipLaunchKernelGGL(k_b_soa_to_aos<float>, dim3(gridsz(cells)), dim3(256), 0, b->stream,
-                         (const float *)b->qbuf[b->cur] + 6 * b->qoff[box], b->stage, cells, base, n);
+  const long long n = b->ncell[box];
+  const bool f32 = b->precision == EVH_FP32_MIXED;
+  const void *src = f32 ? (const void *)((const float *)b->qbuf[b->cur] + 6 * b->qoff[box])
+                        : (const void *)((const double *)b->qbuf[b->cur] + 6 * b->qoff[box]);
+  hipStream_t s = b->stream;
+  BHIP(b->xfer.download(q, n, s, [=](double *stage, long long cells, long long base) {
+    if (f32)
+      hipLaunchKernelGGL(k_b_soa_to_aos<float>, dim3(gridsz(cells)), dim3(256), 0, s, (const float *)src, stage, cells,
+                         base, n);
     else
-      hipLaunchKernelGGL(k_b_soa_to_aos<double>, dim3(gridsz(cells)), dim3(256), 0, b->stream,
-                         (const double *)b->qbuf[b->cur] + 6 * b->qoff[box], b->stage, cells, base, n);
-    BHIP(hipGetLastError());
-    BHIP(hipMemcpyAsync(q + 6 * base, b->stage, 8 * 6 * cells, hipMemcpyDeviceToHost, b->stream));
-    BHIP(hipStreamSynchronize(b->stream));
-  }
+      hipLaunchKernelGGL(k_b_soa_to_aos<double>, dim3(gridsz(cells)), dim3(256), 0, s, (const double *)src, stage,
+                         cells, base, n);
+    return hipGetLastError();
+  }));
   return EVH_OK;
 }
 

@@ -7,6 +7,7 @@
 
 #include "../../include/evora_hip.h"
 #include "physics.h"
+#include "transfer.h"
 
 namespace evh {
 
@@ -44,9 +45,8 @@ struct evh_ctx {
   // staged path arrays (allocated on first use)
   double *w = nullptr, *evo = nullptr, *s = nullptr;
 
-  // transfer staging
-  double *stage = nullptr;
-  size_t stage_elems = 0;
+  // host transfers (pinned, double-buffered chunks)
+  evh::Xfer xfer;
 
   // reductions
   evh::Status *st = nullptr;      // device
