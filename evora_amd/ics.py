@@ -207,6 +207,36 @@ def cosmo_field(params: HydroParams, seed=2024, slope=-2.5, rms=0.5, floor=0.05,
     return prim_to_q(d, rho, vx * scale, vy * scale, vz * scale, rho * temp)
 
 
+# ----------------------------------------------------------------- thermal conduction
+def conduction_params(n=24, **kw) -> HydroParams:
+    """Hot gas with thermal conduction on (simplekappa), HLLC, minmod."""
+    base = dict(nx=n, ny=n, nz=n, boundtype=2, solver=2, limiter=1, cfl=0.4, atomic=True,
+                tstart=0.0, tend=1e30, conduction=True, simplekappa=True)
+    base.update(kw)
+    return HydroParams(**base)
+
+
+def conduction_blob(params: HydroParams, seed=11, t_bg=1000.0, t_blob=3000.0, radius=0.2):
+    """rho = 1 with 1% noise, T = t_bg with a Gaussian hot blob of peak
+    t_blob at the box centre (p = rho T, units of temp0 = 1), and velocity
+    noise of 1% of the background sound speed.  Hot enough that the
+    reference's thermal step (timestep.f90:72-104) keeps the explicit
+    conduction stable."""
+    d = params.derive()
+    x, y, z = _centres(d)
+    shape = (d.nz, d.ny, d.nx)
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.01 * rng.uniform(-1, 1, shape)
+    xc = 0.5 * (params.xmin + params.xmax)
+    yc = 0.5 * (params.ymin + params.ymax)
+    zc = 0.5 * (params.zmin + params.zmax)
+    r2 = (x[None, None, :] - xc) ** 2 + (y[None, :, None] - yc) ** 2 + (z[:, None, None] - zc) ** 2
+    temp = t_bg + (t_blob - t_bg) * np.exp(-r2 / radius ** 2)
+    cs = math.sqrt(d.g * t_bg)
+    v = [0.01 * cs * rng.uniform(-1, 1, shape) for _ in range(3)]
+    return prim_to_q(d, rho, v[0], v[1], v[2], rho * temp)
+
+
 # ----------------------------------------------------------------- small parity states
 def random_smooth(params: HydroParams, seed=0, amp=0.3, jumps=True):
     """A small random state with smooth modes and (optionally) a contact and a

@@ -44,6 +44,9 @@ class HydroParams:
     tstart: float = 0.0
     tend: float = 1.0
     nsnap: int = 100  # output count over [tstart, tend] (parameter.f90:55)
+    conduction: bool = False  # parameter.f90:78
+    c_cond: float = _f32(0.1)
+    simplekappa: bool = False
 
     # ---- parameter.f90:164-265 ----
     def derive(self) -> "Derived":
@@ -66,6 +69,12 @@ class HydroParams:
             zmin, zmax = 0.0, dz
             dz = (zmax - zmin) / self.nz
         g1 = g - 1.0
+        # units.f90:26-65 without supercomoving expansion: l0 = t0 = d0 = cosmo = 1,
+        # m0 = kb (global.f90:103); 37.8 / 40.0 is a single-precision quotient
+        kb = 1.3806200e-16
+        temp0 = (1.0 ** 2.0) / (1.0 ** 2.0) / (1.0 ** 2.0) * kb / kb
+        kappa_coef = 4.6e13 * 1e-20 / _f32(_f32(37.8) / _f32(40.0))
+        kappa0 = (1.0 ** 5.0) * (1.0 ** 3.0) / 1.0 / (1.0 ** 4.0) * 0.5 * kappa_coef
         return Derived(
             nx=self.nx, ny=self.ny, nz=self.nz,
             boundtype=self.boundtype, solver=self.solver, limiter=self.limiter,
@@ -76,6 +85,8 @@ class HydroParams:
             dx=dx, dy=dy, dz=dz, dxinv=1.0 / dx, dyinv=1.0 / dy, dzinv=1.0 / dz,
             volume=(xmax - xmin) * (ymax - ymin) * (zmax - zmin),
             x0=xmin + dx / 2, y0=ymin + dy / 2, z0=zmin + dz / 2,
+            conduction=int(bool(self.conduction)), simplekappa=int(bool(self.simplekappa)),
+            c_cond=self.c_cond, temp0=temp0, kappa0=kappa0,
         )
 
     def to_namelist(self) -> str:
@@ -95,7 +106,8 @@ class HydroParams:
             f" zmin = {r(self.zmin)}, zmax = {r(self.zmax)},\n"
             f" boundtype = {self.boundtype},\n"
             f" dualenergy = {b(self.dualenergy)}, de_thresh = {r(self.de_thresh)},\n"
-            f" tstart = {r(self.tstart)}, tend = {r(self.tend)}, nsnap = {self.nsnap}\n"
+            f" tstart = {r(self.tstart)}, tend = {r(self.tend)}, nsnap = {self.nsnap},\n"
+            f" conduction = {b(self.conduction)}, c_cond = {r(self.c_cond)}, simplekappa = {b(self.simplekappa)}\n"
             "/\n"
         )
 
@@ -129,6 +141,11 @@ class Derived:
     x0: float
     y0: float
     z0: float
+    conduction: int = 0
+    simplekappa: int = 0
+    c_cond: float = 0.1
+    temp0: float = 1.0
+    kappa0: float = 0.0
 
     @property
     def ncells(self) -> int:

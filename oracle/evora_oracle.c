@@ -502,6 +502,89 @@ int ev_sweep(const ev_params *p, const double *w, double *q, const double *evo, 
 }
 
 /* ---- main.f90:109-155 for pure hydro, with updatetime (timestep.f90:163-169) ---- */
+/* ---- timestep.f90:72-104  thermalstep: dt = min(c_cond dx^2 min(rho/kappa) / temp0, dt).
+ * pressure(q) is evaluated on q itself, as the reference passes q(:,ix,iy,iz)
+ * (it rewrites q only with dual energy, which the conduction path refuses). */
+int ev_thermalstep(const ev_params *p, double *q, const double *w, double *dt) {
+  if (p->dualenergy) return EV_ERR_CONFIG;
+  double v = 1e99;
+  int iz;
+#pragma omp parallel for reduction(min : v) schedule(static)
+  for (iz = 1; iz <= p->nz; iz++)
+    for (int iy = 1; iy <= p->ny; iy++)
+      for (int ix = 1; ix <= p->nx; ix++) {
+        double *qc = &q[qidx(p, ix, iy, iz)];
+        double press;
+        pressure(p, qc, &press);
+        const double temp = press / qc[0] * p->temp0;  /* comp.f90:99 */
+        const double kappa = p->kappa0 * pow(temp, 2.5);
+        const double r = w[widx(p, ix, iy, iz)] / kappa;
+        v = r < v ? r : v;  /* min(w(1)/kappa, v) */
+      }
+  v = p->c_cond * (p->dx * p->dx) * v / p->temp0;
+  if (v < *dt) *dt = v;
+  if (*dt <= 0.0) return EV_ERR_DT;
+  return EV_OK;
+}
+
+/* temperature of a w cell (comp.f90:96-100 without chemistry) */
+static inline double wtemp(const ev_params *p, const double *wc) { return wc[4] / wc[0] * p->temp0; }
+
+/* conduction.f90:126-144 with simplekappa: j = -(1/g1) (tr - tl) / delta */
+static inline double tflux(const ev_params *p, double tl, double tr, double deltainv) {
+  const double kappa_eff = 1.0 / p->g1;
+  const double dtempdx = (tr - tl) * deltainv;
+  return -kappa_eff * dtempdx;
+}
+
+/* conduction.f90:146-162 */
+static inline void tintegrate(const ev_params *p, double jl, double jr, double *q, double delta) {
+  const double de = delta * (jl - jr);
+  q[4] = q[4] + de;
+  q[5] = q[5] + p->g1 * de / pow(q[0], p->g1);
+}
+
+/* ---- conduction.f90:26-123  thermalconduction: x, y, z sweeps of the
+ * temperature flux, all from the same w, accumulated into q(5), q(6).
+ * Only simplekappa: the reference's other branch (conduction.f90:135-141)
+ * reads dtempdx before assigning it. */
+int ev_thermalconduction(const ev_params *p, double *q, const double *w, double dt) {
+  if (!p->simplekappa || p->dualenergy) return EV_ERR_CONFIG;
+  const long sx = NW, sy = (long)NW * (p->nx + 4), sz = (long)NW * (p->nx + 4) * (p->ny + 4);
+  const int nd[3] = {p->nx, p->ny, p->nz};
+  const double dinv[3] = {p->dxinv, p->dyinv, p->dzinv};
+  const long step[3] = {sx, sy, sz};
+  for (int d = 0; d < 3; d++) {
+    if (nd[d] == 1) continue;
+    const double delta = dt * dinv[d];
+    /* pencils along d; the two other indices run over the interior */
+    const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+    const int na = nd[a], nb = nd[b];
+    int ib;
+#pragma omp parallel for schedule(static)
+    for (ib = 1; ib <= nb; ib++)
+      for (int ia = 1; ia <= na; ia++) {
+        int idx[3];
+        idx[a] = ia;
+        idx[b] = ib;
+        idx[d] = 0;
+        const double *w0 = &w[widx(p, idx[0], idx[1], idx[2])];
+        double templ = wtemp(p, w0);
+        double tempr = wtemp(p, w0 + step[d]);
+        double jr = tflux(p, templ, tempr, dinv[d]);
+        for (int i = 1; i <= nd[d]; i++) {
+          idx[d] = i;
+          const double jl = jr;
+          templ = tempr;
+          tempr = wtemp(p, &w[widx(p, idx[0], idx[1], idx[2])] + step[d]);
+          jr = tflux(p, templ, tempr, dinv[d]);
+          tintegrate(p, jl, jr, &q[qidx(p, idx[0], idx[1], idx[2])], delta);
+        }
+      }
+  }
+  return EV_OK;
+}
+
 int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, double *t, double tend,
             double *dt, int *go) {
   int err = ev_primitive(p, w, q);
@@ -510,6 +593,10 @@ int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, do
   *dt = 1e99;
   err = ev_hydrostep(p, w, dt);
   if (err) return err;
+  if (p->conduction) { /* main.f90:121 */
+    err = ev_thermalstep(p, q, w, dt);
+    if (err) return err;
+  }
   if (tend - *t <= *dt) {
     *dt = tend - *t;
     *go = 0;
@@ -519,7 +606,13 @@ int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, do
     err = ev_compevo(p, w, evo, s, *dt);
     if (err) return err;
   }
-  return ev_sweep(p, w, q, evo, s, *dt);
+  err = ev_sweep(p, w, q, evo, s, *dt);
+  if (err || !p->conduction) return err;
+  /* main.f90:170-175 */
+  err = ev_primitive(p, w, q);
+  if (err) return err;
+  ev_bound(p, w);
+  return ev_thermalconduction(p, q, w, *dt);
 }
 
 int ev_run(const ev_params *p, double *q, int nsteps, double *t, double tend, double *dt_last, int *done) {

@@ -10,6 +10,10 @@
  *   source/boundary.f90:26-113   bound
  *   source/timestep.f90:26-57    hydrostep
  *   source/timestep.f90:147-171  updatetime (non-supercomoving branch)
+ *   source/timestep.f90:72-104   thermalstep
+ *   source/conduction.f90:26-164 thermalconduction, thermalflux (simplekappa),
+ *                                thermalintegration; comp.f90:87-102 press2temp
+ *                                (without chemistry)
  *   source/snapshot.f90:159-193  evolution sums
  *
  * Array layouts are the reference's Fortran layouts (component fastest):
@@ -57,6 +61,9 @@ typedef struct {
   double g, g1, g1inv, gg1, cfl, de_thresh1;
   double dx, dy, dz, dxinv, dyinv, dzinv;
   double volume;
+  /* thermal conduction (parameter.f90:77-78; units.f90:50,63) */
+  int conduction, simplekappa;
+  double c_cond, temp0, kappa0;
 } ev_params;
 
 /* cell of the last error: ix, iy, iz (Fortran indices) */
@@ -78,6 +85,8 @@ int ev_sweep(const ev_params *p, const double *w, double *q, const double *evo, 
 void ev_flux(const ev_params *p, const double *w, double *f, int dir);
 void ev_state(const ev_params *p, const double *w, double *q);
 void ev_statex(const ev_params *p, const double *w, double *qx, double s, double sm, int dir);
+int ev_thermalstep(const ev_params *p, double *q, const double *w, double *dt);
+int ev_thermalconduction(const ev_params *p, double *q, const double *w, double dt);
 
 /* One pass of main.f90:109-155 for pure hydro; t, dt, go are in/out. */
 int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s,

@@ -51,6 +51,8 @@ class EvParams(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
         ("dxinv", C.c_double), ("dyinv", C.c_double), ("dzinv", C.c_double),
         ("volume", C.c_double),
+        ("conduction", C.c_int), ("simplekappa", C.c_int),
+        ("c_cond", C.c_double), ("temp0", C.c_double), ("kappa0", C.c_double),
     ]
 
 
@@ -84,6 +86,8 @@ def lib():
             "ev_q2w": [P, D, D],
             "ev_step": [P, D, D, D, D, D, C.c_double, D, C.POINTER(C.c_int)],
             "ev_run": [P, D, C.c_int, D, C.c_double, D, C.POINTER(C.c_int)],
+            "ev_thermalstep": [P, D, D, D],
+            "ev_thermalconduction": [P, D, D, C.c_double],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args

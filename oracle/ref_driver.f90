@@ -9,8 +9,9 @@
 !   ranges      lx=1..nx etc. as decomposition does for mpisize=1
 !               (parallel.f90:33-64)
 !   arrays      q, w, evo, s allocated and zeroed as main.f90:72-77
-!   loop        primitive, bound, dt=1d99, hydrostep, updatetime, compevo,
-!               sweep                                  (main.f90:109-155)
+!   loop        primitive, bound, dt=1d99, hydrostep, [thermalstep],
+!               updatetime, compevo, sweep, [primitive, bound,
+!               thermalconduction]   (main.f90:109-175; [] with conduction)
 !
 ! usage: evref <parafile> <mode> <in.bin> <out.bin> [nsteps]
 !   mode run    : in = q(nq,nx,ny,nz) f64; nsteps passes; out = q
@@ -48,6 +49,7 @@ program evref
 
   call readparameter()
   call compparameter()
+  call compunits()     ! main.f90:62
 
   lx  = 1      ; mx  = nx
   lx1 = lx - 1 ; lx2 = lx - 2 ; lx0 = lx + 1
@@ -120,6 +122,7 @@ program evref
      call bound(w)
      dt = 1d99
      call hydrostep(w)
+     if (conduction) call thermalstep(q, w)    ! main.f90:121
      call updatetime(go, dump, k4)
      if (secondorder) call compevo(w, evo, s)
      if (trim(mode) .eq. 'stage') then
@@ -129,6 +132,11 @@ program evref
         write (u) s
      endif
      call sweep(w, q, evo, s)
+     if (conduction) then                       ! main.f90:170-175
+        call primitive(w, q)
+        call bound(w)
+        call thermalconduction(q, w)
+     endif
      write (*, '(A,I6,2ES25.16)') 'STEP ', k, t, dt
      if (evolve) then
         tsnap = (t - tstart) / (tend - tstart) * nsnap

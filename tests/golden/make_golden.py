@@ -91,6 +91,18 @@ def main():
     runs["cosmo24_de/q"] = qf
     runs["cosmo24_de/t_dt"] = np.array(steps)
     index["cosmo24_de"] = dict(kind="run", params=p.__dict__, nsteps=10)
+    # thermal conduction (simplekappa) on a hot blob, 5 steps: thermalstep
+    # limits dt, then primitive, bound and thermalconduction follow the sweep
+    for solver, bt in ((2, 2), (1, 1), (2, 3)):
+        p = ics.conduction_params(16, solver=solver, boundtype=bt)
+        q0 = ics.conduction_blob(p)
+        qf, steps, log = O.ref_run(p, q0, 5)
+        assert qf is not None and len(steps) == 5, log[-400:]
+        name = f"cond16_s{solver}_b{bt}"
+        runs[name + "/q0"] = q0
+        runs[name + "/q"] = qf
+        runs[name + "/t_dt"] = np.array(steps)
+        index[name] = dict(kind="run", params=p.__dict__, nsteps=5)
     # configured CFL numbers at which the reference scheme itself stops on a
     # negative pressure (3-D unsplit, CFL limit taken per direction)
     aborts = {

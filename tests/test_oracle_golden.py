@@ -29,7 +29,8 @@ def test_stage_bitwise(name, stage_fixtures, golden_index):
     assert dt == stage_fixtures[name + "/dt"][0]
 
 
-RUNS = ["sod1000", "run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3", "cosmo24_de"]
+RUNS = ["sod1000", "run16_s2_l1_b2", "run16_s1_l3_b1", "run16_s2_l2_b3", "cosmo24_de",
+        "cond16_s2_b2", "cond16_s1_b1", "cond16_s2_b3"]
 ABORTS = ["blast32_hll", "kh24_cfl08", "cosmo16_nodual"]
 
 

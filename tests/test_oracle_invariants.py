@@ -187,3 +187,65 @@ def test_negative_pressure_code():
     with pytest.raises(O.OracleError) as ei:
         O.Oracle(d).run(q, 1)
     assert ei.value.code == 1 and ei.value.cell == (4, 3, 2)
+
+
+# ---------------------------------------------------------------- thermal conduction
+def test_conduction_linear_profile_closed_form():
+    """conduction.f90 with simplekappa: j = -(1/g1) dT/dx at each face.  For
+    a temperature linear in x (exactly representable values) every face flux
+    is the same number, so interior cells do not change at all; the mirror
+    ghosts of `bound` make the two end faces carry no flux, so the hot end
+    loses exactly the flux of its inner face and the cold end gains it."""
+    p = HydroParams(nx=8, ny=4, nz=4, boundtype=1, atomic=True, conduction=True, simplekappa=True)
+    d = p.derive()
+    o = O.Oracle(d)
+    shape = (d.nz, d.ny, d.nx)
+    q = np.zeros(shape + (6,))
+    q[..., 0] = 1.0
+    temp = 64.0 - 4.0 * np.arange(d.nx)  # 64, 60, ..., 36
+    q[..., 4] = np.broadcast_to(temp, shape) / d.g1  # p = rho T (temp0 = 1), u = 0
+    q[..., 5] = np.broadcast_to(temp, shape)  # p / rho**g1 with rho = 1
+    w = O.new_w(d)
+    o.primitive(w, q)
+    o.bound(w)
+    q1 = q.copy()
+    dt = 1e-3
+    O._check(o.L.ev_thermalconduction(O.C.byref(o.p), O._ptr(q1), O._ptr(w), dt))
+    de = q1[..., 4] - q[..., 4]
+    assert np.all(de[..., 1:-1] == 0.0)
+    j = -(1.0 / d.g1) * (-4.0) * d.dxinv  # the flux of every inner face
+    assert de[..., 0] == pytest.approx(dt * d.dxinv * (0.0 - j), rel=1e-14)
+    assert de[..., -1] == pytest.approx(dt * d.dxinv * (j - 0.0), rel=1e-14)
+    assert np.all(de[..., 0] < 0) and np.all(de[..., -1] > 0)  # heat flows from hot to cold
+
+
+def test_conduction_periodic_conserves_energy():
+    p = ics.conduction_params(16, boundtype=2)
+    q = ics.conduction_blob(p)
+    o = O.Oracle(p.derive())
+    s0 = o.sums(q)
+    o.run(q, 5)
+    s1 = o.sums(q)
+    assert s1[0] == pytest.approx(s0[0], rel=1e-14)
+    assert s1[1] == pytest.approx(s0[1], rel=1e-13)
+
+
+def test_conduction_uniform_state_unchanged():
+    p = ics.conduction_params(12, boundtype=1)
+    d = p.derive()
+    q = np.zeros((12, 12, 12, 6))
+    q[..., 0] = 1.0
+    q[..., 4] = 1000.0 / d.g1
+    q[..., 5] = 1000.0
+    q1 = q.copy()
+    O.Oracle(d).run(q1, 3)
+    assert np.array_equal(q1, q)
+
+
+def test_conduction_refuses_full_kappa():
+    """The reference's kappa_eff with simplekappa off reads dtempdx before
+    assigning it (conduction.f90:134-141): that branch is refused."""
+    p = ics.conduction_params(8, simplekappa=False)
+    q = ics.conduction_blob(p)
+    with pytest.raises(O.OracleError):
+        O.Oracle(p.derive()).run(q, 1)
