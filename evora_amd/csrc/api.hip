@@ -190,6 +190,11 @@ int validate(const evh_config *c) {
   if (c->nx < 2 || (!oned && c->ny < 2)) return EVH_ERR_CONFIG;
   if (c->precision != EVH_FP64 && c->precision != EVH_FP32_MIXED) return EVH_ERR_CONFIG;
   if (c->precision == EVH_FP32_MIXED && !fused_supported(*c)) return EVH_ERR_CONFIG;
+  // conduction: simplekappa only (the reference's other branch reads an
+  // unassigned dtempdx, conduction.f90:134-141), no dual energy (thermalstep
+  // evaluates pressure on q a second time, timestep.f90:87), 3-D grids
+  if (c->conduction && (!c->simplekappa || c->dualenergy || !fused_supported(*c) || !(c->temp0 > 0.0)))
+    return EVH_ERR_CONFIG;
   return EVH_OK;
 }
 
@@ -213,7 +218,10 @@ KParams make_kparams(const evh_config &c) {
   const double delta = 2.0 / 3.0 - c.g1;
   k.g1_third = std::fabs(delta) <= 1e-6 ? 1 : 0;
   k.g1_dln2 = delta * 0.69314718055994530942;
-  k.pad_ = 0;
+  k.conduction = c.conduction;
+  k.c_cond = c.c_cond;
+  k.temp0 = c.temp0;
+  k.kappa0 = c.kappa0;
   return k;
 }
 
@@ -316,6 +324,7 @@ int evh_upload_q(evh_ctx *h, const double *q) {
     return hipGetLastError();
   }));
   h->v_valid = false;
+  h->tv_valid = false;
   h->w_valid = false;
   return EVH_OK;
 }
@@ -420,6 +429,7 @@ int evh_sweep(evh_ctx *h, double dt) {
     }
     h->w_valid = false;
     h->bound_done = false;
+    h->tv_valid = false;
     return check_status(h);
   }
   if (!h->v_valid) return fail(h, EVH_ERR_STATE, "sweep before primitive");
@@ -433,6 +443,45 @@ int evh_sweep(evh_ctx *h, double dt) {
   }
   h->cur = 1 - h->cur;
   h->v_valid = true;
+  h->tv_valid = false;
+  return EVH_OK;
+}
+
+// timestep.f90:72-104: dt = min(c_cond dx^2 min(rho/kappa) / temp0, dt)
+int evh_thermalstep(evh_ctx *h, double *dt) {
+  if (!h || !dt) return EVH_ERR_STATE;
+  if (!h->cfg.conduction) return fail(h, EVH_ERR_STATE, "thermalstep without conduction");
+  if (!h->tv_valid) {
+    HIP_TRY(launch_thermal_min(h));
+    h->tv_valid = true;
+  }
+  int r = check_status(h);
+  if (r) return r;
+  if (h->st_host->next_code) {
+    h->v_valid = h->tv_valid = false;
+    return fail(h, (int)h->st_host->next_code, "thermalstep", (long long)h->st_host->next_cell);
+  }
+  double v;
+  const unsigned long long bits = h->st_host->tvmin_bits;
+  std::memcpy(&v, &bits, sizeof v);
+  v = h->cfg.c_cond * (h->cfg.dx * h->cfg.dx) * v / h->cfg.temp0;
+  *dt = v < *dt ? v : *dt;
+  if (*dt <= 0.0) return fail(h, EVH_ERR_DT);
+  return EVH_OK;
+}
+
+// conduction.f90:26-123 on the state after the sweep, with the primitive and
+// bound of main.f90:171-172 folded into the kernel
+int evh_thermalconduction(evh_ctx *h, double dt) {
+  if (!h) return EVH_ERR_STATE;
+  if (!h->cfg.conduction) return fail(h, EVH_ERR_STATE, "thermalconduction without conduction");
+  HIP_TRY(launch_reset_status(h, false));
+  HIP_TRY(launch_conduction(h, dt));
+  h->cur = 1 - h->cur;
+  h->v_valid = true;
+  h->tv_valid = true;
+  h->w_valid = false;
+  h->bound_done = false;
   return EVH_OK;
 }
 
@@ -443,13 +492,16 @@ int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go) {
   if ((r = evh_bound(h))) return r;
   *dt = 1e99;  // main.f90:118
   if ((r = evh_hydrostep(h, dt))) return r;
+  if (h->cfg.conduction && (r = evh_thermalstep(h, dt))) return r;  // main.f90:121
   if (tend - *t <= *dt) {  // timestep.f90:163-166
     *dt = tend - *t;
     *go = 0;
   }
   *t = *t + *dt;
   if (h->cfg.secondorder && (r = evh_compevo(h, *dt))) return r;
-  return evh_sweep(h, *dt);
+  if ((r = evh_sweep(h, *dt))) return r;
+  if (h->cfg.conduction) return evh_thermalconduction(h, *dt);  // main.f90:170-175
+  return EVH_OK;
 }
 
 int evh_run(evh_ctx *h, int nsteps, double *t, double tend, double *dt, int *done) {

@@ -56,6 +56,7 @@ struct evh_ctx {
 
   // step state
   bool v_valid = false;  // st->vmin_bits holds the CFL min of the current q
+  bool tv_valid = false; // st->tvmin_bits holds the conduction limit of the current q
   bool w_valid = false;  // staged: w holds primitive of current q (bound applied?)
   bool bound_done = false;
   double dt_evo = 0.0;
@@ -91,6 +92,8 @@ hipError_t launch_fused_step(evh_ctx *h, double dt);  // cur -> other, CFL of th
 hipError_t launch_fused_f64(evh_ctx *h, double dt);
 hipError_t launch_fused_f32(evh_ctx *h, double dt);
 hipError_t launch_fused_exact(evh_ctx *h, double dt);  // FP64, reference arithmetic
+hipError_t launch_thermal_min(evh_ctx *h);              // conduction limit of cur q
+hipError_t launch_conduction(evh_ctx *h, double dt);    // cur -> other, limits of the result
 bool fused_supported(const evh_config &c);
 
 // KParams of a configuration (api.hip)

@@ -37,7 +37,8 @@ struct KParams {
   double kin_frac64;  // FP32-mixed: kin/E above which p is recovered in FP64
   double g1_dln2;     // (2/3 - g1) ln 2
   int g1_third;       // |2/3 - g1| <= 1e-6: powers of g1 via the cube root
-  int pad_;
+  int conduction;     // thermal conduction (simplekappa) on
+  double c_cond, temp0, kappa0;
 };
 
 // Device status block: first error (code, linear cell) and min reductions.
@@ -48,6 +49,8 @@ struct Status {
   unsigned long long vmin_bits;   // CFL min of dx/(|u|+a) (positive double bits)
   unsigned long long next_code;   // error found while computing the next step's primitive
   unsigned long long next_cell;
+  unsigned long long tvmin_bits;  // conduction limit: min of rho / kappa (positive double bits)
+  unsigned long long pad_;
 };
 
 __device__ __forceinline__ void report(Status *st, int code, unsigned long long cell) {

@@ -30,6 +30,8 @@ module evora_hip
      real(c_double) :: g, g1, g1inv, gg1, cfl, de_thresh1
      real(c_double) :: dx, dy, dz, dxinv, dyinv, dzinv
      real(c_double) :: volume, mach_fp64
+     integer(c_int) :: conduction, simplekappa
+     real(c_double) :: c_cond, temp0, kappa0
   end type evh_config
 
   interface
@@ -38,6 +40,18 @@ module evora_hip
        type(evh_config), intent(in) :: cfg
        type(c_ptr), intent(out) :: h
      end function evh_create
+
+     integer(c_int) function evh_thermalstep(h, dt) bind(c, name='evh_thermalstep')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(inout) :: dt
+     end function evh_thermalstep
+
+     integer(c_int) function evh_thermalconduction(h, dt) bind(c, name='evh_thermalconduction')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: dt
+     end function evh_thermalconduction
 
      integer(c_int) function evh_destroy(h) bind(c, name='evh_destroy')
        import :: c_int, c_ptr

@@ -127,9 +127,13 @@ contains
   subroutine hip_create()
     use global
     type(evh_config) :: cfg
-    if (gravity .or. chemistry .or. conduction .or. supercomoving .or. tempfloor .or. jeansfloor &
+    if (gravity .or. chemistry .or. supercomoving .or. tempfloor .or. jeansfloor &
          .or. cooldt .or. noncie .or. parallel .or. .not. hydro) then
-       print *, 'ERROR: EVORA_BACKEND=hip runs pure hydro on one rank only'
+       print *, 'ERROR: EVORA_BACKEND=hip runs hydro (and thermal conduction) on one rank only'
+       call cleanstop
+    endif
+    if (conduction .and. (.not. simplekappa .or. dualenergy)) then
+       print *, 'ERROR: EVORA_BACKEND=hip runs thermal conduction with simplekappa and without dualenergy only'
        call cleanstop
     endif
     cfg%nx = nx ; cfg%ny = ny ; cfg%nz = nz
@@ -145,11 +149,31 @@ contains
     cfg%dxinv = dxinv ; cfg%dyinv = dyinv ; cfg%dzinv = dzinv
     cfg%volume = volume
     cfg%mach_fp64 = 5.0d0
+    cfg%conduction = merge(1, 0, conduction)
+    cfg%simplekappa = merge(1, 0, simplekappa)
+    cfg%c_cond = c_cond ; cfg%temp0 = temp0 ; cfg%kappa0 = kappa0
     call hip_check(evh_create(cfg, ctx))
     ! 3-D grids start in the fused form; the exact form on request (FP64)
     if (exact .and. precision .eq. EVH_FP64 .and. .not. onedimension .and. .not. twodimension) &
          call hip_check(evh_set_mode(ctx, EVH_MODE_FUSED_EXACT))
   end subroutine hip_create
+
+  ! main.f90:181-195 after this step (t already advanced by updatetime):
+  ! download q when the driver is about to read it; evolution takes its
+  ! sums on the device and needs no q
+  subroutine download_if_read(q)
+    use global
+    real(8), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+    real(8) :: tsnap_next
+    logical :: need_q
+    need_q = last_step
+    tsnap_next = (t - tstart) / (tend - tstart) * nsnap
+    if ((cut .or. slice .or. bov) .and. tsnap_next .ge. dble(k3s)) then
+       need_q = .true.
+       k3s = k3s + 1
+    endif
+    if (need_q) call hip_check(evh_download_q(ctx, q))
+  end subroutine download_if_read
 
 end module hydro_dispatch_state
 
@@ -228,8 +252,6 @@ subroutine sweep(w, q, evo, s)
   real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2)   :: w
   real(8), dimension(nw,lx1:mx1,ly1:my1,lz1:mz1)   :: evo
   real(8), dimension(nw,3,lx1:mx1,ly1:my1,lz1:mz1) :: s
-  real(8) :: tsnap_next
-  logical :: need_q
 
   call decide_backend()
   if (backend .eq. 0) then
@@ -237,15 +259,8 @@ subroutine sweep(w, q, evo, s)
      return
   endif
   call hip_check(evh_sweep(ctx, dt))
-  ! main.f90:181-195 after this step (t already advanced by updatetime);
-  ! evolution takes its sums on the device and needs no q
-  need_q = last_step
-  tsnap_next = (t - tstart) / (tend - tstart) * nsnap
-  if ((cut .or. slice .or. bov) .and. tsnap_next .ge. dble(k3s)) then
-     need_q = .true.
-     k3s = k3s + 1
-  endif
-  if (need_q) call hip_check(evh_download_q(ctx, q))
+  ! with conduction the step ends in thermalconduction, which downloads
+  if (.not. conduction) call download_if_read(q)
 end subroutine sweep
 
 ! snapshot.f90:159-193.  Before the state is on the device (the call of
@@ -281,3 +296,39 @@ subroutine evolution(k2, q)
   endif
   k2 = k2 + 1
 end subroutine evolution
+
+! timestep.f90:72-104 (main.f90:121)
+subroutine thermalstep(q, w)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2) :: w
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_thermalstep(q, w)
+     return
+  endif
+  call hip_check(evh_thermalstep(ctx, dt))
+  last_step = (tend - t .le. dt)  ! dt may have shrunk: the test of timestep.f90:163 again
+end subroutine thermalstep
+
+! conduction.f90:26-123 (main.f90:174).  main.f90:171-172 call primitive and
+! bound before it; on the device those are folded into the conduction kernel
+! (the dispatch's primitive and bound find the state already current).
+subroutine thermalconduction(q, w)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+  real(8), dimension(nw,lx2:mx2,ly2:my2,lz2:mz2) :: w
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_thermalconduction(q, w)
+     return
+  endif
+  call hip_check(evh_thermalconduction(ctx, dt))
+  call download_if_read(q)
+end subroutine thermalconduction

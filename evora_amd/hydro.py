@@ -34,6 +34,7 @@ ERR_NAMES = {
 EXPORTS = (
     "evh_create", "evh_destroy", "evh_set_mode", "evh_upload_q", "evh_download_q",
     "evh_primitive", "evh_bound", "evh_hydrostep", "evh_compevo", "evh_sweep",
+    "evh_thermalstep", "evh_thermalconduction",
     "evh_step", "evh_run", "evh_sums", "evh_download_w", "evh_download_evo_s",
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
     "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_device_bytes",
@@ -56,6 +57,8 @@ class EvhConfig(C.Structure):
         ("dxinv", C.c_double), ("dyinv", C.c_double), ("dzinv", C.c_double),
         ("volume", C.c_double),
         ("mach_fp64", C.c_double),
+        ("conduction", C.c_int), ("simplekappa", C.c_int),
+        ("c_cond", C.c_double), ("temp0", C.c_double), ("kappa0", C.c_double),
     ]
 
 
@@ -85,6 +88,7 @@ def load_library():
             "evh_upload_q": [P, D], "evh_download_q": [P, D],
             "evh_primitive": [P], "evh_bound": [P], "evh_hydrostep": [P, D],
             "evh_compevo": [P, C.c_double], "evh_sweep": [P, C.c_double],
+            "evh_thermalstep": [P, D], "evh_thermalconduction": [P, C.c_double],
             "evh_step": [P, D, C.c_double, D, C.POINTER(C.c_int)],
             "evh_run": [P, C.c_int, D, C.c_double, D, C.POINTER(C.c_int)],
             "evh_sums": [P, D], "evh_download_w": [P, D], "evh_download_evo_s": [P, D, D],
@@ -208,6 +212,16 @@ class Hydro:
     def sweep(self, dt: float):
         """muscl.f90:171-274"""
         self._check(self.L.evh_sweep(self.h, dt))
+
+    def thermalstep(self, dt: float = 1e99) -> float:
+        """timestep.f90:72-104: returns min(c_cond dx^2 min(rho/kappa)/temp0, dt)"""
+        x = C.c_double(dt)
+        self._check(self.L.evh_thermalstep(self.h, C.byref(x)))
+        return x.value
+
+    def thermalconduction(self, dt: float):
+        """conduction.f90:26-123 (with the primitive and bound before it)"""
+        self._check(self.L.evh_thermalconduction(self.h, dt))
 
     def step(self, t: float, tend: float):
         """One pass of main.f90:109-155; returns (t, dt, go)."""

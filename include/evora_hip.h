@@ -55,6 +55,10 @@ typedef struct evh_config {
   double dx, dy, dz, dxinv, dyinv, dzinv;
   double volume;
   double mach_fp64; /* FP32_MIXED: pressure from E in FP64 where Mach > mach_fp64 (e.g. 5) */
+  /* thermal conduction (parameter.f90:77-78, units.f90:50,63): only
+   * simplekappa, without dual energy, on 3-D grids */
+  int conduction, simplekappa;
+  double c_cond, temp0, kappa0;
 } evh_config;
 
 typedef struct evh_ctx evh_ctx;
@@ -84,8 +88,15 @@ int evh_hydrostep(evh_ctx *h, double *dt);       /* hydrostep(w)       timestep.
                                                     dt = min(cfl*v, dt), in/out              */
 int evh_compevo(evh_ctx *h, double dt);          /* compevo(w,evo,s)   muscl.f90:45-113       */
 int evh_sweep(evh_ctx *h, double dt);            /* sweep(w,q,evo,s)   muscl.f90:171-274      */
+/* thermal conduction (conduction on in the config): */
+int evh_thermalstep(evh_ctx *h, double *dt);     /* thermalstep(q,w)   timestep.f90:72-104:
+                                                    dt = min(c_cond dx^2 min(rho/kappa)/temp0, dt) */
+int evh_thermalconduction(evh_ctx *h, double dt); /* thermalconduction(q,w) conduction.f90:26-123,
+                                                    with the primitive and bound main.f90:171-172
+                                                    calls before it; simplekappa only        */
 
-/* One pass of main.f90:109-155 for pure hydro on the device, with
+/* One pass of main.f90:109-155 on the device (with thermalstep and the
+ * conduction of main.f90:121,170-175 when conduction is on), with
  * updatetime (timestep.f90:163-169): t, dt, go in/out.  evh_run repeats it
  * nsteps times (or until go turns 0), syncing only for dt. */
 int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go);
