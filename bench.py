@@ -17,11 +17,13 @@ Printed: one JSON line with the headline number (FP64 fused path) and,
 alongside, the roofline of the dominant kernel (HIP-event timing of each
 sweep launch on the library's stream), the FP64 fused step in its exact form
 (the reference's own arithmetic, bit-identical results), the FP32-mixed and
-batched forms, and the reference's CPU hot path timed on the host's cores on
-a bounded sample.  Unless --no-accuracy is given, each FP32-mixed configuration (the
-blast with HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50
-steps, the cosmological-like field 256^3 for 100 steps, and the batched
-ensemble) is also advanced in FP64 for the same number of steps, and the
+batched forms, the step with thermal conduction on (whole steps against the
+hydro step alone, and the conduction kernel, on a 512^3 hot blob), and the
+reference's CPU hot path timed on the host's cores on a bounded sample.
+Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
+HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50 steps, the
+cosmological-like field 256^3 for 100 steps, and the batched ensemble) is
+also advanced in FP64 for the same number of steps, and the
 relative L1 errors of rho, p and |v| and the mass and energy drift of both
 forms are reported under forms.*.accuracy_vs_fp64 (outside every timed
 region).
@@ -43,6 +45,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"fp64": 96.0, "fp32_mixed": 48.0}  # read + write 6 conserved values
+COND_BYTES_PER_CELL = {"fp64": 64.0, "fp32_mixed": 32.0}  # read 6 conserved values, write 2
 
 
 def log(msg):
@@ -115,6 +118,7 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
         h.synchronize()
         wall = time.perf_counter() - t0
         kms, launches = h.kernel_stats()
+        cms, claunches = h.conduction_stats()
         sums = h.sums()
         t_dn = time.perf_counter()
         q = h.download_q() if keep_q or time_xfer else None
@@ -126,6 +130,7 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
         ms_per_step=1e3 * wall / steps,
         kernel_ms=kms / max(launches, 1),
         launches=launches,
+        cond_ms=cms / max(claunches, 1),
         t_end=t,
         sums0=sums0,
         sums=sums,
@@ -137,6 +142,35 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
     log(f"{precision_name}: {res['value'] / 1e9:.2f} G cell-updates/s, {res['ms_per_step']:.3f} ms/step, "
         f"kernel {res['kernel_ms']:.3f} ms")
     return res
+
+
+def run_conduction(n, steps, warmup, fp64_only=False):
+    """The step with thermal conduction on (hydro step, thermalstep and the
+    conduction pass of main.f90:121,170-175) on an n^3 hot blob, FP64 fused
+    (and FP32-mixed), against the hydro step alone on the same state.  The
+    conduction kernel is timed per launch with events on the library's
+    stream; its algorithmic bytes are the compulsory ones of the pass (read
+    the 6 conserved values, write q(5) and q(6))."""
+    from evora_amd import ics
+
+    p = ics.conduction_params(n, boundtype=2)
+    d = p.derive()
+    dh = ics.conduction_params(n, boundtype=2, conduction=False).derive()
+    log(f"building {n}^3 conduction initial condition")
+    q0 = ics.conduction_blob(p)
+    out = {"workload": f"hot blob {n}^3 (T 1000, peak 3000), periodic, HLLC, minmod, CFL 0.4, simplekappa"}
+    for prec in ("fp64",) if fp64_only else ("fp64", "fp32_mixed"):
+        r = run_gpu(d, q0, steps, warmup, prec)
+        rh = run_gpu(dh, q0, steps, warmup, prec)
+        ms = r["cond_ms"]
+        nbytes = COND_BYTES_PER_CELL[prec]
+        ach = nbytes * d.ncells / (ms * 1e-3) / 1e9
+        out[prec] = dict(value=r["value"], ms_per_step=r["ms_per_step"], hydro_only_ms_per_step=rh["ms_per_step"],
+                         sweep_kernel_ms=r["kernel_ms"], conduction_kernel_ms=ms, conduction_bytes_per_cell=nbytes,
+                         conduction_GBs=ach, conduction_hbm_frac=ach / HBM_PEAK_GBS)
+        log(f"conduction {prec}: kernel {ms:.3f} ms ({ach:.0f} GB/s), step {r['ms_per_step']:.3f} ms "
+            f"vs hydro alone {rh['ms_per_step']:.3f} ms")
+    return out
 
 
 def compare_forms(d, r64, r32):
@@ -337,6 +371,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-mixed against FP64 error and drift runs")
+    ap.add_argument("--no-conduction", action="store_true", help="skip the thermal-conduction form")
+    ap.add_argument("--conduction-only", action="store_true",
+                    help="run only the thermal-conduction form and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--batch-boxes", type=int, default=48)
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -351,6 +388,9 @@ def main():
 
     from evora_amd import ics
 
+    if args.conduction_only:
+        print(json.dumps(run_conduction(args.n, args.steps, args.warmup, args.fp64_only)), flush=True)
+        return
     p = ics.blast_params(args.n, solver=2, cfl=args.cfl)
     d = p.derive()
     log(f"building {args.n}^3 blast initial condition")
@@ -388,6 +428,8 @@ def main():
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         forms["batched"] = run_batch(args.batch_boxes, args.batch_steps, 2,
                                      log_path=os.path.join(ROOT, "gpurun_out", "batch_autotune.log"), accuracy=acc)
+    if not args.no_conduction:
+        forms["conduction"] = run_conduction(args.n, args.steps, args.warmup, args.fp64_only)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
     if os.path.exists(tpath):

@@ -33,7 +33,8 @@ UNITS = [
     ("fused64.o", "fused.hip", ["-DEVH_FUSED_PREC=64"]),
     ("fused32.o", "fused.hip", ["-DEVH_FUSED_PREC=32"]),
     ("fused64x.o", "fused.hip", ["-DEVH_FUSED_PREC=64", "-DEVH_FUSED_EXACT=1", "-ffp-contract=off"]),
-    ("conduction.o", "conduction.hip", ["-ffp-contract=off"]),
+    ("conduction.o", "conduction.hip", []),
+    ("conductionx.o", "conduction.hip", ["-DEVH_COND_EXACT=1", "-ffp-contract=off"]),
     ("batch64.o", "batch.hip", ["-DEVH_FUSED_PREC=64"]),
     ("batch32.o", "batch.hip", ["-DEVH_FUSED_PREC=32"]),
 ]

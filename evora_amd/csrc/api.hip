@@ -141,13 +141,21 @@ int grid_of(long long work) {
 bool mixed(const evh_ctx *h) { return h->cfg.precision == EVH_FP32_MIXED; }
 
 void accumulate_timing(evh_ctx *h) {
-  if (!h->tev_pending) return;
   float ms = 0.f;
-  if (hipEventSynchronize(h->tev[1]) == hipSuccess && hipEventElapsedTime(&ms, h->tev[0], h->tev[1]) == hipSuccess) {
-    h->kernel_ms += ms;
-    h->kernel_launches += 1;
+  if (h->tev_pending) {
+    if (hipEventSynchronize(h->tev[1]) == hipSuccess && hipEventElapsedTime(&ms, h->tev[0], h->tev[1]) == hipSuccess) {
+      h->kernel_ms += ms;
+      h->kernel_launches += 1;
+    }
+    h->tev_pending = false;
   }
-  h->tev_pending = false;
+  if (h->tcv_pending) {
+    if (hipEventSynchronize(h->tcv[1]) == hipSuccess && hipEventElapsedTime(&ms, h->tcv[0], h->tcv[1]) == hipSuccess) {
+      h->cond_ms += ms;
+      h->cond_launches += 1;
+    }
+    h->tcv_pending = false;
+  }
 }
 
 // Sync and surface device-side errors.  Errors stay sticky until read here.
@@ -264,10 +272,16 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
   for (auto &ev : h->tev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
+  for (auto &ev : h->tcv)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
   const long long n = h->grid.n();
   for (int b = 0; b < 2; b++) {
     if ((e = hipMalloc(&h->qbuf[b], h->qelem * 6 * n)) != hipSuccess) return bail(e);
     h->dev_bytes += (long long)h->qelem * 6 * n;
+  }
+  if (h->cfg.conduction) {
+    if ((e = hipMalloc(&h->tbuf, h->qelem * n)) != hipSuccess) return bail(e);
+    h->dev_bytes += (long long)h->qelem * n;
   }
   if ((e = h->xfer.init(n)) != hipSuccess) return bail(e);
   if ((e = hipMalloc(&h->st, sizeof(Status))) != hipSuccess) return bail(e);
@@ -284,6 +298,7 @@ int evh_destroy(evh_ctx *h) {
   if (!h) return EVH_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
   for (auto p : h->qbuf) if (p) hipFree(p);
+  if (h->tbuf) hipFree(h->tbuf);
   if (h->w) hipFree(h->w);
   if (h->evo) hipFree(h->evo);
   if (h->s) hipFree(h->s);
@@ -293,6 +308,7 @@ int evh_destroy(evh_ctx *h) {
   if (h->partials) hipFree(h->partials);
   for (auto ev : h->events) if (ev) hipEventDestroy(ev);
   for (auto ev : h->tev) if (ev) hipEventDestroy(ev);
+  for (auto ev : h->tcv) if (ev) hipEventDestroy(ev);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return EVH_OK;
@@ -304,7 +320,7 @@ int evh_set_mode(evh_ctx *h, int mode) {
   if (mode != EVH_MODE_STAGED && !fused_supported(h->cfg)) return fail(h, EVH_ERR_CONFIG, "fused needs a 3-D grid");
   if (mode != EVH_MODE_FUSED && mixed(h)) return fail(h, EVH_ERR_CONFIG, "staged and exact kernels are FP64");
   h->mode = mode;
-  h->v_valid = false;
+  h->v_valid = h->tv_valid = h->t_valid = false;  // limits and T are formed in the mode's arithmetic
   h->w_valid = false;
   return EVH_OK;
 }
@@ -325,6 +341,7 @@ int evh_upload_q(evh_ctx *h, const double *q) {
   }));
   h->v_valid = false;
   h->tv_valid = false;
+  h->t_valid = false;
   h->w_valid = false;
   return EVH_OK;
 }
@@ -430,6 +447,7 @@ int evh_sweep(evh_ctx *h, double dt) {
     h->w_valid = false;
     h->bound_done = false;
     h->tv_valid = false;
+    h->t_valid = false;
     return check_status(h);
   }
   if (!h->v_valid) return fail(h, EVH_ERR_STATE, "sweep before primitive");
@@ -444,6 +462,7 @@ int evh_sweep(evh_ctx *h, double dt) {
   h->cur = 1 - h->cur;
   h->v_valid = true;
   h->tv_valid = false;
+  h->t_valid = h->tbuf != nullptr;  // the sweep stored T of the new state
   return EVH_OK;
 }
 
@@ -475,11 +494,18 @@ int evh_thermalstep(evh_ctx *h, double *dt) {
 int evh_thermalconduction(evh_ctx *h, double dt) {
   if (!h) return EVH_ERR_STATE;
   if (!h->cfg.conduction) return fail(h, EVH_ERR_STATE, "thermalconduction without conduction");
+  if (!h->t_valid) HIP_TRY(launch_temp(h));
   HIP_TRY(launch_reset_status(h, false));
+  accumulate_timing(h);
+  if (h->timing) HIP_TRY(hipEventRecord(h->tcv[0], h->stream));
   HIP_TRY(launch_conduction(h, dt));
-  h->cur = 1 - h->cur;
+  if (h->timing) {
+    HIP_TRY(hipEventRecord(h->tcv[1], h->stream));
+    h->tcv_pending = true;
+  }
   h->v_valid = true;
   h->tv_valid = true;
+  h->t_valid = false;  // E changed
   h->w_valid = false;
   h->bound_done = false;
   return EVH_OK;
@@ -635,8 +661,8 @@ int evh_set_timing(evh_ctx *h, int on) {
   int r = check_status(h);
   if (r) return r;
   h->timing = on != 0;
-  h->kernel_ms = 0.0;
-  h->kernel_launches = 0;
+  h->kernel_ms = h->cond_ms = 0.0;
+  h->kernel_launches = h->cond_launches = 0;
   return EVH_OK;
 }
 
@@ -646,6 +672,15 @@ int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches) {
   if (r) return r;
   *total_ms = h->kernel_ms;
   *launches = h->kernel_launches;
+  return EVH_OK;
+}
+
+int evh_conduction_stats(evh_ctx *h, double *total_ms, long long *launches) {
+  if (!h || !total_ms || !launches) return EVH_ERR_STATE;
+  int r = check_status(h);
+  if (r) return r;
+  *total_ms = h->cond_ms;
+  *launches = h->cond_launches;
   return EVH_OK;
 }
 

@@ -42,6 +42,11 @@ struct evh_ctx {
   int cur = 0;
   size_t qelem = 8;  // bytes per stored value
 
+  // temperature of each cell of the current q (conduction on only): stored
+  // by the fused sweep, read by the conduction pass; storage precision
+  void *tbuf = nullptr;
+  bool t_valid = false;
+
   // staged path arrays (allocated on first use)
   double *w = nullptr, *evo = nullptr, *s = nullptr;
 
@@ -75,6 +80,10 @@ struct evh_ctx {
   bool tev_pending = false;
   double kernel_ms = 0.0;
   long long kernel_launches = 0;
+  hipEvent_t tcv[2] = {nullptr, nullptr};  // the same for conduction passes
+  bool tcv_pending = false;
+  double cond_ms = 0.0;
+  long long cond_launches = 0;
 };
 
 namespace evh {
@@ -92,8 +101,13 @@ hipError_t launch_fused_step(evh_ctx *h, double dt);  // cur -> other, CFL of th
 hipError_t launch_fused_f64(evh_ctx *h, double dt);
 hipError_t launch_fused_f32(evh_ctx *h, double dt);
 hipError_t launch_fused_exact(evh_ctx *h, double dt);  // FP64, reference arithmetic
+// conduction kernels (conduction.hip)
 hipError_t launch_thermal_min(evh_ctx *h);              // conduction limit of cur q
-hipError_t launch_conduction(evh_ctx *h, double dt);    // cur -> other, limits of the result
+hipError_t launch_temp(evh_ctx *h);                     // tbuf from cur q
+hipError_t launch_conduction(evh_ctx *h, double dt);    // cur q in place (tbuf valid), limits of the result
+hipError_t launch_thermal_min_exact(evh_ctx *h);        // the same with the reference's arithmetic
+hipError_t launch_temp_exact(evh_ctx *h);
+hipError_t launch_conduction_exact(evh_ctx *h, double dt);
 bool fused_supported(const evh_config &c);
 
 // KParams of a configuration (api.hip)

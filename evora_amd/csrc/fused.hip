@@ -41,11 +41,12 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
 // (512 / WPE VGPRs per lane); the tuning knob next to the tile height TY.
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-    k_fused(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st) {
+    k_fused(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
+            S *__restrict__ tout) {
   __shared__ core::Smem<T, TY> sm;
   const int lin = xcd_tile(blockIdx.x, G.ntiles);
   if (lin < 0) return;  // padding block (whole block exits before any barrier)
-  core::march<T, S, SOLVER, LIMITER, TY, kExact>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm);
+  core::march<T, S, SOLVER, LIMITER, TY, kExact>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm, tout);
 }
 
 // primitive + CFL min over q without materialising w (first step, after
@@ -85,7 +86,8 @@ void launch_ty(evh_ctx *h, double dt) {
   const Geo g = make_geo(h, TY);
   const int blocks = kXcds * ((g.ntiles + kXcds - 1) / kXcds);
   hipLaunchKernelGGL((k_fused<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
-                     h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st);
+                     h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
+                     (S *)h->tbuf);
 }
 
 // tuning variants: tile height 8 or 12, waves per SIMD (FP64: 2 or 3, FP32: 3 or 4)

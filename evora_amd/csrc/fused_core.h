@@ -141,6 +141,11 @@ struct Phys<false> {
   __device__ static __forceinline__ double cfl_cell(const KParams &P, const T w[5], T rinv) {
     return fast::cfl_cell(P, w, rinv);
   }
+  // temperature p / rho * temp0 (for thermal conduction)
+  template <typename T>
+  __device__ static __forceinline__ T temp(const KParams &P, const T w[5], T rinv) {
+    return w[4] * rinv * T(P.temp0);
+  }
 };
 
 // The reference's HLL and HLLC (riemann.f90:47-153) with its expressions
@@ -276,6 +281,9 @@ struct Phys<true> {
   __device__ static __forceinline__ double cfl_cell(const KParams &P, const double w[5], double) {
     return evh::cfl_cell(P, w);
   }
+  __device__ static __forceinline__ double temp(const KParams &P, const double w[5], double) {
+    return w[4] / w[0] * P.temp0;
+  }
 };
 
 // raw q at (gx, gy, gz), ghost-mapped (the index rules of `bound`)
@@ -334,10 +342,13 @@ struct Smem {
 // One tile `lin` (x fastest, then y, then z chunk) of the box whose q starts
 // at element qoff of qin_all/qout_all.  vmin_dst receives the CFL minimum of
 // the tile's new state; errors carry the box index above bit 40 of the cell.
+// With tout (single box only), the temperature of each new cell is stored
+// there too, for the conduction pass that follows the sweep.
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT>
 __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__restrict__ qin_all,
                                       S *__restrict__ qout_all, long long qoff, int box, double dt, Status *st,
-                                      unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm) {
+                                      unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm,
+                                      S *__restrict__ tout = nullptr) {
   using TS = TileShape<TY>;
   constexpr int NI = TS::NI, NR1 = TS::NR1, EW = TS::EW, EA = TS::EA;
   T(&W)[5][EA] = sm.W;
@@ -504,6 +515,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
           const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
           if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
           vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
+          if (tout) tout[c] = S(Phys<EXACT>::temp(P, wn, rn));
         }
 #pragma unroll
         for (int m = 0; m < 6; m++) CR[6 + m][tid] = fz[m];

@@ -37,7 +37,8 @@ EXPORTS = (
     "evh_thermalstep", "evh_thermalconduction",
     "evh_step", "evh_run", "evh_sums", "evh_download_w", "evh_download_evo_s",
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
-    "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_device_bytes",
+    "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_conduction_stats",
+    "evh_device_bytes",
     "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
     "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
     "evh_batch_get_tuning", "evh_batch_autotune", "evh_batch_last_error",
@@ -100,6 +101,7 @@ def load_library():
             "evh_set_tuning": [P, C.c_int, C.c_int, C.c_int],
             "evh_get_tuning": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "evh_kernel_stats": [P, D, C.POINTER(C.c_longlong)],
+            "evh_conduction_stats": [P, D, C.POINTER(C.c_longlong)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -281,6 +283,12 @@ class Hydro:
         """(total ms, launches) of the sweep kernels since set_timing(True)."""
         ms, n = C.c_double(), C.c_longlong()
         self._check(self.L.evh_kernel_stats(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def conduction_stats(self):
+        """(total ms, launches) of the conduction kernel since set_timing(True)."""
+        ms, n = C.c_double(), C.c_longlong()
+        self._check(self.L.evh_conduction_stats(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
     def device_bytes(self) -> int:

@@ -126,10 +126,13 @@ int evh_set_tuning(evh_ctx *h, int tile_ty, int wpe_hi, int zchunk);
 int evh_get_tuning(evh_ctx *h, int *tile_ty, int *wpe_hi, int *zchunk);
 
 /* Kernel timing (no counterpart in the reference; for benchmarks): when on,
- * every fused or staged sweep launch is bracketed by events on the context's
- * stream and its duration is added to a running total at the next sync. */
+ * every fused or staged sweep launch and every conduction-kernel launch is
+ * bracketed by events on the context's stream and its duration is added to a
+ * running total at the next sync. */
 int evh_set_timing(evh_ctx *h, int on);
 int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches);
+/* the same for the conduction kernel of evh_thermalconduction */
+int evh_conduction_stats(evh_ctx *h, double *total_ms, long long *launches);
 
 /* Bytes of device memory the context holds. */
 long long evh_device_bytes(evh_ctx *h);
