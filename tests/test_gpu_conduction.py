@@ -9,7 +9,6 @@ exact form to 1e-12 (the conduction limit takes T**2.5 and the entropy
 update q(1)**g1 from the device power, which may differ from the host
 libm's in the last bit); FP32-mixed relative L1 of rho and p <= 1e-5.
 """
-import os
 
 import numpy as np
 import pytest
@@ -116,3 +115,52 @@ def test_conduction_dropin(exact):
     tol = 1e-12 if exact else 1e-10
     for c in range(6):
         assert rel_l1(qh[..., c], qc[..., c]) <= tol, c
+
+
+@pytest.mark.parametrize("mode", [hydro.MODE_FUSED, hydro.MODE_FUSED_EXACT])
+def test_conduction_linear_profile_closed_form(mode):
+    """A temperature linear in x with mirror ghosts (conduction.f90 with
+    simplekappa): inner cells do not change, the hot end loses exactly the
+    flux of its inner face (-(1/g1) dT/dx) and the cold end gains it.  Called
+    on an uploaded state, so T comes from the stand-alone temperature pass."""
+    from evora_amd.params import HydroParams
+    p = HydroParams(nx=8, ny=4, nz=4, boundtype=1, atomic=True, conduction=True, simplekappa=True)
+    d = p.derive()
+    shape = (d.nz, d.ny, d.nx)
+    q = np.zeros(shape + (6,))
+    q[..., 0] = 1.0
+    temp = 64.0 - 4.0 * np.arange(d.nx)
+    q[..., 4] = np.broadcast_to(temp, shape) / d.g1
+    q[..., 5] = np.broadcast_to(temp, shape)
+    dt = 1e-3
+    with hydro.Hydro(d, mode=mode) as h:
+        h.upload_q(q)
+        h.thermalconduction(dt)
+        q1 = h.download_q()
+    de = q1[..., 4] - q[..., 4]
+    assert np.all(de[..., 1:-1] == 0.0)
+    j = -(1.0 / d.g1) * (-4.0) * d.dxinv
+    assert de[..., 0] == pytest.approx(dt * d.dxinv * (0.0 - j), rel=1e-14)
+    assert de[..., -1] == pytest.approx(dt * d.dxinv * (j - 0.0), rel=1e-14)
+    assert np.array_equal(q1[..., :4], q[..., :4])
+
+
+def test_conduction_fullsize_conservation_and_forms():
+    """256^3 periodic hot blob, 4 steps: mass and energy conserved to
+    roundoff, and the fast form within 1e-12 (relative L-infinity per
+    component) of the reference-arithmetic form."""
+    p = ics.conduction_params(256, boundtype=2)
+    d = p.derive()
+    q0 = ics.conduction_blob(p)
+    out = {}
+    for mode in (hydro.MODE_FUSED, hydro.MODE_FUSED_EXACT):
+        with hydro.Hydro(d, mode=mode) as h:
+            h.upload_q(q0)
+            s0 = h.sums()
+            h.run(4)
+            s1 = h.sums()
+            out[mode] = h.download_q()
+        assert s1[0] == pytest.approx(s0[0], rel=1e-13)
+        assert s1[1] == pytest.approx(s0[1], rel=1e-13)
+    for c in range(6):
+        assert rel_linf(out[hydro.MODE_FUSED][..., c], out[hydro.MODE_FUSED_EXACT][..., c]) <= 1e-12, c
