@@ -30,10 +30,12 @@
 ! about to read q: when snapshot will run (the tsnap test of
 ! main.f90:192-195, mirrored with a shadow counter) and on the last step
 ! (tend - t <= dt, the test of timestep.f90:163).  w, evo and s are never
-! filled on the host; pure hydro never reads them there.  Physics that
-! update q on the host between steps (gravity, chemistry, floors,
-! conduction, supercomoving expansion) and multi-rank runs are refused on
-! the GPU backend.
+! filled on the host; pure hydro never reads them there.  Thermal
+! conduction (thermalstep, thermalconduction with simplekappa, without dual
+! energy) runs on the device too.  Physics that update q on the host between
+! steps (gravity, chemistry, floors, supercomoving expansion), conduction
+! with the full kappa or with dual energy, and multi-rank runs are refused
+! on the GPU backend.
 !-----------------------------------------------------------------------------
 module hydro_dispatch_state
   use iso_c_binding
