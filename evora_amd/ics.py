@@ -238,6 +238,36 @@ def conduction_blob(params: HydroParams, seed=11, t_bg=1000.0, t_blob=3000.0, ra
 
 
 # ----------------------------------------------------------------- small parity states
+def jeans_params(n=32, **kw) -> HydroParams:
+    """Self-gravitating periodic box (gravity on, not comoving: 4 pi G =
+    8.38553911e-7 as in parameter.f90:240), HLLC, minmod, gamma 5/3."""
+    base = dict(nx=n, ny=n, nz=n, boundtype=2, solver=2, limiter=1, cfl=0.4, atomic=True,
+                tstart=0.0, tend=1e30, gravity=True)
+    base.update(kw)
+    return HydroParams(**base)
+
+
+def jeans(params: HydroParams, seed=5, rho0=1e6, jeans_length=0.5, amp=0.1, noise=0.01):
+    """rho0 (1 + amp cos(2 pi x) cos(2 pi y) + noise) with a uniform pressure
+    whose sound speed gives the Jeans length lambda_J = c sqrt(pi / (G rho0))
+    = jeans_length (in box units, 4 pi G = grav): the box-scale mode is
+    gravitationally unstable and collapses, the short ones oscillate.
+    Velocities carry 1% noise of the sound speed."""
+    d = params.derive()
+    x, y, z = _centres(d)
+    shape = (d.nz, d.ny, d.nx)
+    rng = np.random.default_rng(seed)
+    lx, ly = params.xmax - params.xmin, params.ymax - params.ymin
+    mode = np.cos(2 * math.pi * x[None, None, :] / lx) * np.cos(2 * math.pi * y[None, :, None] / ly)
+    rho = rho0 * (1.0 + amp * np.broadcast_to(mode, shape) + noise * rng.uniform(-1, 1, shape))
+    grav = d.grav if d.grav > 0 else 8.38553911e-7
+    cs2 = jeans_length ** 2 * grav * rho0 / (4.0 * math.pi ** 2)
+    pres = np.full(shape, rho0 * cs2 / d.g)
+    cs = math.sqrt(cs2)
+    v = [0.01 * cs * rng.uniform(-1, 1, shape) for _ in range(3)]
+    return prim_to_q(d, rho, v[0], v[1], v[2], pres)
+
+
 def random_smooth(params: HydroParams, seed=0, amp=0.3, jumps=True):
     """A small random state with smooth modes and (optionally) a contact and a
     pressure jump, for one-step parity tests on tiny grids."""

@@ -47,6 +47,10 @@ class HydroParams:
     conduction: bool = False  # parameter.f90:78
     c_cond: float = _f32(0.1)
     simplekappa: bool = False
+    gravity: bool = False  # parameter.f90:57
+    supercomoving: bool = False  # parameter.f90:61-64
+    omega0: float = 1.0
+    fb: float = 1.0
 
     # ---- parameter.f90:164-265 ----
     def derive(self) -> "Derived":
@@ -75,6 +79,11 @@ class HydroParams:
         temp0 = (1.0 ** 2.0) / (1.0 ** 2.0) / (1.0 ** 2.0) * kb / kb
         kappa_coef = 4.6e13 * 1e-20 / _f32(_f32(37.8) / _f32(40.0))
         kappa0 = (1.0 ** 5.0) * (1.0 ** 3.0) / 1.0 / (1.0 ** 4.0) * 0.5 * kappa_coef
+        # gravity and the Poisson solver (parameter.f90:235-241,254-263); pi is
+        # the single-precision literal of global.f90:98 held in a real(8)
+        pi = _f32(3.1415927)
+        grav = (1.5 * self.omega0 if self.supercomoving else 8.38553911e-7) if self.gravity else 0.0
+        kfx, kfy, kfz = 2.0 * pi / self.nx, 2.0 * pi / self.ny, 2.0 * pi / self.nz
         return Derived(
             nx=self.nx, ny=self.ny, nz=self.nz,
             boundtype=self.boundtype, solver=self.solver, limiter=self.limiter,
@@ -87,6 +96,11 @@ class HydroParams:
             x0=xmin + dx / 2, y0=ymin + dy / 2, z0=zmin + dz / 2,
             conduction=int(bool(self.conduction)), simplekappa=int(bool(self.simplekappa)),
             c_cond=self.c_cond, temp0=temp0, kappa0=kappa0,
+            gravity=int(bool(self.gravity)), supercomoving=int(bool(self.supercomoving)),
+            grav=grav, fb=self.fb, h2=dx ** 2.0 / 4.0,
+            kfx2=kfx * 0.5, kfy2=kfy * 0.5, kfz2=kfz * 0.5,
+            nyqx=max(self.nx // 2, 1), nyqy=max(self.ny // 2, 1), nyqz=max(self.nz // 2, 1),
+            dx2inv=0.5 / dx, dy2inv=0.5 / dy, dz2inv=0.5 / dz,
         )
 
     def to_namelist(self) -> str:
@@ -107,7 +121,9 @@ class HydroParams:
             f" boundtype = {self.boundtype},\n"
             f" dualenergy = {b(self.dualenergy)}, de_thresh = {r(self.de_thresh)},\n"
             f" tstart = {r(self.tstart)}, tend = {r(self.tend)}, nsnap = {self.nsnap},\n"
-            f" conduction = {b(self.conduction)}, c_cond = {r(self.c_cond)}, simplekappa = {b(self.simplekappa)}\n"
+            f" conduction = {b(self.conduction)}, c_cond = {r(self.c_cond)}, simplekappa = {b(self.simplekappa)},\n"
+            f" gravity = {b(self.gravity)}, supercomoving = {b(self.supercomoving)}, omega0 = {r(self.omega0)},"
+            f" fb = {r(self.fb)}\n"
             "/\n"
         )
 
@@ -146,6 +162,20 @@ class Derived:
     c_cond: float = 0.1
     temp0: float = 1.0
     kappa0: float = 0.0
+    gravity: int = 0
+    supercomoving: int = 0
+    grav: float = 0.0
+    fb: float = 1.0
+    h2: float = 0.0
+    kfx2: float = 0.0
+    kfy2: float = 0.0
+    kfz2: float = 0.0
+    nyqx: int = 1
+    nyqy: int = 1
+    nyqz: int = 1
+    dx2inv: float = 0.0
+    dy2inv: float = 0.0
+    dz2inv: float = 0.0
 
     @property
     def ncells(self) -> int:

@@ -585,6 +585,156 @@ int ev_thermalconduction(const ev_params *p, double *q, const double *w, double 
   return EV_OK;
 }
 
+/* ---- gravitation.f90:26-88 (gravpotential, multigreen), :172-254 (gravintegration, grav_force) ---- */
+
+/* The transforms of fft.f90:58-80 (FFTW, complex to complex, in place, no
+ * normalisation): sign -1 forward, +1 backward, along every axis with
+ * n > 1.  Restated as the defining sums a_k = sum_j a_j exp(sign 2 pi i jk/n),
+ * evaluated directly along each axis (O(n^2) per line, for test sizes), with
+ * the twiddles of the true pi. */
+static const double kPi = 3.14159265358979323846;
+
+static void dft_axis(double *re, double *im, int n, long stride, long nlines, const long *line0, int sign) {
+  double *c = malloc(sizeof(double) * (size_t)n), *sn = malloc(sizeof(double) * (size_t)n);
+  double *br = malloc(sizeof(double) * (size_t)n), *bi = malloc(sizeof(double) * (size_t)n);
+  for (int j = 0; j < n; j++) {
+    c[j] = cos(2.0 * kPi * j / n);
+    sn[j] = sign * sin(2.0 * kPi * j / n);
+  }
+  for (long l = 0; l < nlines; l++) {
+    const long b = line0[l];
+    for (int k = 0; k < n; k++) {
+      double sr = 0.0, si = 0.0;
+      for (int j = 0; j < n; j++) {
+        const int m = (int)(((long)j * k) % n);
+        const double xr = re[b + stride * j], xi = im[b + stride * j];
+        sr += xr * c[m] - xi * sn[m];
+        si += xr * sn[m] + xi * c[m];
+      }
+      br[k] = sr;
+      bi[k] = si;
+    }
+    for (int k = 0; k < n; k++) {
+      re[b + stride * k] = br[k];
+      im[b + stride * k] = bi[k];
+    }
+  }
+  free(c);
+  free(sn);
+  free(br);
+  free(bi);
+}
+
+void ev_dft3(const ev_params *p, double *re, double *im, int sign) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  long *l0 = malloc(sizeof(long) * (size_t)((long)nx * ny + (long)nx * nz + (long)ny * nz));
+  long m = 0;
+  if (nx > 1) {
+    m = 0;
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++) l0[m++] = (long)nx * (j + (long)ny * k);
+    dft_axis(re, im, nx, 1, m, l0, sign);
+  }
+  if (ny > 1) {
+    m = 0;
+    for (int k = 0; k < nz; k++)
+      for (int i = 0; i < nx; i++) l0[m++] = i + (long)nx * ny * k;
+    dft_axis(re, im, ny, nx, m, l0, sign);
+  }
+  if (nz > 1) {
+    m = 0;
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) l0[m++] = i + (long)nx * j;
+    dft_axis(re, im, nz, (long)nx * ny, m, l0, sign);
+  }
+  free(l0);
+}
+
+/* comp.f90:241-252 */
+static double kvalue(int i, int n, int nyq) { return i <= nyq ? (double)(i - 1) : -(double)(n - (i - 1)); }
+
+/* gravitation.f90:55-89: phi(k) = phi(k) / n3 * green(k), green = -h2 / sum sin^2(kf/2 k), 0 at k = 0.
+ * `phi / dble(n3)` divides a complex by a real promoted to complex; the
+ * reference's compiler forms the textbook quotient (a c + b d) / (c^2 + d^2),
+ * (b c - a d) / (c^2 + d^2) with d = 0, which is not a / c bit for bit when
+ * n3 is not a power of two, and that is the form restated here. */
+void ev_multigreen(const ev_params *p, double *re, double *im) {
+  const double n3 = (double)p->nx * p->ny * p->nz;
+  for (int iz = 1; iz <= p->nz; iz++) {
+    const double sz = sin(p->kfz2 * kvalue(iz, p->nz, p->nyqz));
+    const double sin2z = sz * sz;
+    for (int iy = 1; iy <= p->ny; iy++) {
+      const double sy = sin(p->kfy2 * kvalue(iy, p->ny, p->nyqy));
+      const double sin2y = sy * sy;
+      for (int ix = 1; ix <= p->nx; ix++) {
+        const double sx = sin(p->kfx2 * kvalue(ix, p->nx, p->nyqx));
+        const double sin2x = sx * sx;
+        const double green = (ix == 1 && iy == 1 && iz == 1) ? 0.0 : -p->h2 / (sin2x + sin2y + sin2z);
+        const long c = (long)(ix - 1) + (long)p->nx * ((long)(iy - 1) + (long)p->ny * (iz - 1));
+        const double a = re[c], b = im[c], den = n3 * n3 + 0.0 * 0.0;
+        re[c] = (a * n3 + b * 0.0) / den * green;
+        im[c] = (b * n3 - a * 0.0) / den * green;
+      }
+    }
+  }
+}
+
+/* gravitation.f90:26-52 for q(nq,...): the real part of the potential into
+ * phi(1:nx,1:ny,1:nz) (grav_force reads dble(phi) only).  cosmo is the
+ * expansion factor of the supercomoving source. */
+int ev_gravpotential(const ev_params *p, const double *q, double *phi, double cosmo) {
+  const long n = (long)p->nx * p->ny * p->nz;
+  double *re = malloc(sizeof(double) * (size_t)n), *im = calloc((size_t)n, sizeof(double));
+  if (!re || !im) {
+    free(re);
+    free(im);
+    return EV_ERR_CONFIG;
+  }
+  if (p->supercomoving) {
+    for (long c = 0; c < n; c++) re[c] = cosmo * p->grav * (q[NQ * c] / p->fb - 1.0);
+  } else {
+    double meanrho = 0.0;
+    for (long c = 0; c < n; c++) meanrho += q[NQ * c];
+    meanrho = meanrho / (double)n;
+    for (long c = 0; c < n; c++) re[c] = p->grav * (q[NQ * c] - meanrho);
+  }
+  ev_dft3(p, re, im, -1);
+  ev_multigreen(p, re, im);
+  ev_dft3(p, re, im, +1);
+  memcpy(phi, re, sizeof(double) * (size_t)n);
+  free(re);
+  free(im);
+  return EV_OK;
+}
+
+/* gravitation.f90:172-254, one rank (periodic in every direction) */
+int ev_gravintegration(const ev_params *p, double *q, const double *phi, double dt) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+#define PHI(i, j, k) phi[(long)((i) - 1) + (long)nx * ((long)((j) - 1) + (long)ny * ((k) - 1))]
+  for (int iz = 1; iz <= nz; iz++)
+    for (int iy = 1; iy <= ny; iy++)
+      for (int ix = 1; ix <= nx; ix++) {
+        double f[3];
+        if (nx == 1) f[0] = 0;
+        else if (ix == 1) f[0] = -(PHI(2, iy, iz) - PHI(nx, iy, iz)) * p->dx2inv;
+        else if (ix == nx) f[0] = -(PHI(1, iy, iz) - PHI(nx - 1, iy, iz)) * p->dx2inv;
+        else f[0] = -(PHI(ix + 1, iy, iz) - PHI(ix - 1, iy, iz)) * p->dx2inv;
+        if (ny == 1) f[1] = 0;
+        else if (iy == 1) f[1] = -(PHI(ix, 2, iz) - PHI(ix, ny, iz)) * p->dy2inv;
+        else if (iy == ny) f[1] = -(PHI(ix, 1, iz) - PHI(ix, ny - 1, iz)) * p->dy2inv;
+        else f[1] = -(PHI(ix, iy + 1, iz) - PHI(ix, iy - 1, iz)) * p->dy2inv;
+        if (nz == 1) f[2] = 0;
+        else if (iz == 1) f[2] = -(PHI(ix, iy, 2) - PHI(ix, iy, nz)) * p->dz2inv;
+        else if (iz == nz) f[2] = -(PHI(ix, iy, 1) - PHI(ix, iy, nz - 1)) * p->dz2inv;
+        else f[2] = -(PHI(ix, iy, iz + 1) - PHI(ix, iy, iz - 1)) * p->dz2inv;
+        double *qc = &q[qidx(p, ix, iy, iz)];
+        qc[4] = qc[4] + dt * (qc[1] * f[0] + qc[2] * f[1] + qc[3] * f[2]);
+        for (int d = 0; d < 3; d++) qc[1 + d] = qc[1 + d] + dt * qc[0] * f[d];
+      }
+#undef PHI
+  return EV_OK;
+}
+
 int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, double *t, double tend,
             double *dt, int *go) {
   int err = ev_primitive(p, w, q);
@@ -607,7 +757,16 @@ int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, do
     if (err) return err;
   }
   err = ev_sweep(p, w, q, evo, s, *dt);
-  if (err || !p->conduction) return err;
+  if (err) return err;
+  if (p->gravity) { /* main.f90:157-164, one rank, not comoving (cosmo = 1) */
+    double *phi = malloc(sizeof(double) * (size_t)((long)p->nx * p->ny * p->nz));
+    if (!phi) return EV_ERR_CONFIG;
+    err = ev_gravpotential(p, q, phi, 1.0);
+    if (!err) err = ev_gravintegration(p, q, phi, *dt);
+    free(phi);
+    if (err) return err;
+  }
+  if (!p->conduction) return EV_OK;
   /* main.f90:170-175 */
   err = ev_primitive(p, w, q);
   if (err) return err;

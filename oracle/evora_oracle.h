@@ -14,6 +14,9 @@
  *   source/conduction.f90:26-164 thermalconduction, thermalflux (simplekappa),
  *                                thermalintegration; comp.f90:87-102 press2temp
  *                                (without chemistry)
+ *   source/gravitation.f90:26-88,172-254 gravpotential (one rank), multigreen,
+ *                                gravintegration, grav_force; fft.f90:58-80
+ *                                as the defining DFT sums; comp.f90:241-252 kvalue
  *   source/snapshot.f90:159-193  evolution sums
  *
  * Array layouts are the reference's Fortran layouts (component fastest):
@@ -64,6 +67,11 @@ typedef struct {
   /* thermal conduction (parameter.f90:77-78; units.f90:50,63) */
   int conduction, simplekappa;
   double c_cond, temp0, kappa0;
+  /* gravity (parameter.f90:235-241,254-263) */
+  int gravity, supercomoving;
+  double grav, fb, h2, kfx2, kfy2, kfz2;
+  int nyqx, nyqy, nyqz;
+  double dx2inv, dy2inv, dz2inv;
 } ev_params;
 
 /* cell of the last error: ix, iy, iz (Fortran indices) */
@@ -88,7 +96,13 @@ void ev_statex(const ev_params *p, const double *w, double *qx, double s, double
 int ev_thermalstep(const ev_params *p, double *q, const double *w, double *dt);
 int ev_thermalconduction(const ev_params *p, double *q, const double *w, double dt);
 
-/* One pass of main.f90:109-155 for pure hydro; t, dt, go are in/out. */
+/* gravity; phi is the real potential (nx, ny, nz), re/im complex fields of that shape */
+void ev_dft3(const ev_params *p, double *re, double *im, int sign);
+void ev_multigreen(const ev_params *p, double *re, double *im);
+int ev_gravpotential(const ev_params *p, const double *q, double *phi, double cosmo);
+int ev_gravintegration(const ev_params *p, double *q, const double *phi, double dt);
+
+/* One pass of main.f90:109-175 (hydro, conduction and non-comoving gravity); t, dt, go are in/out. */
 int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s,
             double *t, double tend, double *dt, int *go);
 /* nsteps passes (stops early when go turns false); returns steps done in *done. */

@@ -53,6 +53,11 @@ class EvParams(C.Structure):
         ("volume", C.c_double),
         ("conduction", C.c_int), ("simplekappa", C.c_int),
         ("c_cond", C.c_double), ("temp0", C.c_double), ("kappa0", C.c_double),
+        ("gravity", C.c_int), ("supercomoving", C.c_int),
+        ("grav", C.c_double), ("fb", C.c_double), ("h2", C.c_double),
+        ("kfx2", C.c_double), ("kfy2", C.c_double), ("kfz2", C.c_double),
+        ("nyqx", C.c_int), ("nyqy", C.c_int), ("nyqz", C.c_int),
+        ("dx2inv", C.c_double), ("dy2inv", C.c_double), ("dz2inv", C.c_double),
     ]
 
 
@@ -88,12 +93,18 @@ def lib():
             "ev_run": [P, D, C.c_int, D, C.c_double, D, C.POINTER(C.c_int)],
             "ev_thermalstep": [P, D, D, D],
             "ev_thermalconduction": [P, D, D, C.c_double],
+            "ev_gravpotential": [P, D, D, C.c_double],
+            "ev_gravintegration": [P, D, D, C.c_double],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
         L.ev_sums.argtypes = [P, D, D]
         L.ev_sums.restype = None
+        for name in ("ev_dft3", "ev_multigreen"):
+            fn = getattr(L, name)
+            fn.argtypes = [P, D, D] + ([C.c_int] if name == "ev_dft3" else [])
+            fn.restype = None
         _lib = L
     return _lib
 
@@ -194,6 +205,28 @@ class Oracle:
         self.L.ev_sums(C.byref(self.p), _ptr(q), _ptr(out))
         return out
 
+    # gravity (gravitation.f90); phi and complex fields have shape (nz, ny, nx)
+    def dft3(self, a, sign):
+        """FFTW's unnormalised transform of fft.f90 (sign -1 forward, +1 backward)."""
+        re = np.ascontiguousarray(a.real, dtype=np.float64)
+        im = np.ascontiguousarray(a.imag, dtype=np.float64)
+        self.L.ev_dft3(C.byref(self.p), _ptr(re), _ptr(im), sign)
+        return re + 1j * im
+
+    def multigreen(self, a):
+        re = np.ascontiguousarray(a.real, dtype=np.float64)
+        im = np.ascontiguousarray(a.imag, dtype=np.float64)
+        self.L.ev_multigreen(C.byref(self.p), _ptr(re), _ptr(im))
+        return re + 1j * im
+
+    def gravpotential(self, q, cosmo=1.0):
+        phi = np.zeros((self.d.nz, self.d.ny, self.d.nx))
+        _check(self.L.ev_gravpotential(C.byref(self.p), _ptr(q), _ptr(phi), cosmo))
+        return phi
+
+    def gravintegration(self, q, phi, dt):
+        _check(self.L.ev_gravintegration(C.byref(self.p), _ptr(q), _ptr(np.ascontiguousarray(phi)), dt))
+
 
 # ---------------------------------------------------------------- reference run
 def have_ref() -> bool:
@@ -236,6 +269,26 @@ def ref_stage(params, q):
     q1 = out[o:o + nq].reshape(d.nz, d.ny, d.nx, 6); o += nq
     dt = out[o]
     return w, evo, s, q1, dt
+
+
+def ref_green(params, a):
+    """The reference's multigreen on the complex field a (nz, ny, nx)."""
+    raw = np.empty(a.shape + (2,))
+    raw[..., 0], raw[..., 1] = a.real, a.imag
+    out, log = run_ref(params, "green", raw, 1)
+    if out is None:
+        raise RuntimeError("reference green failed:\n" + log)
+    out = out.reshape(a.shape + (2,))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def ref_gravint(params, q, phi, dt):
+    """The reference's gravintegration of q (nz, ny, nx, 6) with potential phi and step dt."""
+    data = np.concatenate([np.ravel(q), np.ravel(phi), [dt]])
+    out, log = run_ref(params, "gravint", data, 1)
+    if out is None:
+        raise RuntimeError("reference gravint failed:\n" + log)
+    return out.reshape(q.shape)
 
 
 def ref_run(params, q, nsteps):

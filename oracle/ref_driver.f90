@@ -18,6 +18,11 @@
 !   mode stage  : in = q; one pass; out = w, evo, s, q_after_sweep, dt
 !   mode riemann: in = n records of (wl(5), wr(5), dir); out = f(6) per record
 !   mode slope  : in = n records of (wl(5), w(5), wr(5)); out = s(5) per record
+!   mode green  : in = phi(nx,ny,nz) complex(8); multigreen
+!                 (gravitation.f90:55-89); out = phi
+!   mode gravint: in = q(nq,nx,ny,nz), phi(nx,ny,nz) real(8), dt;
+!                 gravintegration (gravitation.f90:172-198) with phi as the
+!                 real part of the potential; out = q
 ! Each step prints "STEP k t dt" to stdout; the step loop's wall time is
 ! printed as "LOOP_SECONDS s" (system_clock, I/O excluded).  Reference errors print their own
 ! "ERROR ..." lines and `stop` (exit status 0), exactly as in the reference.
@@ -28,7 +33,8 @@ program evref
 
   character(len=256) :: mode, fin, fout, arg
   real(8), allocatable :: q(:,:,:,:), w(:,:,:,:), evo(:,:,:,:), s(:,:,:,:,:)
-  real(8), allocatable :: rec(:)
+  real(8), allocatable :: rec(:), phir(:,:,:), phibuf2(:,:), phibuf4(:,:)
+  complex(8), allocatable :: phic(:,:,:)
   real(8) :: wl(nw), wr(nw), wc(nw), f(nf), sl(nw)
   integer :: nsteps, k, n, i, u, fsize, dir, k4, k2, l, st
   integer(8) :: c0, c1, crate
@@ -85,6 +91,33 @@ program evref
            call comp_slope(wl, wc, wr, sl)
            write (u) sl
         enddo
+     endif
+     close (u)
+     stop
+  endif
+
+  if (trim(mode) .eq. 'green' .or. trim(mode) .eq. 'gravint') then
+     allocate(phic(lx:mx,ly:my,lz:mz), phibuf2(lx:mx,ly:my), phibuf4(lx:mx,ly:my))
+     phibuf2 = 0.0
+     phibuf4 = 0.0
+     open (newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+     if (trim(mode) .eq. 'green') then
+        read (u) phic
+        call multigreen(phic)
+     else
+        allocate(q(nq,lx:mx,ly:my,lz:mz), phir(lx:mx,ly:my,lz:mz))
+        read (u) q
+        read (u) phir
+        read (u) dt
+        phic = cmplx(phir, 0.0d0, kind=8)
+        call gravintegration(q, phic, phibuf2, phibuf4)
+     endif
+     close (u)
+     open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+     if (trim(mode) .eq. 'green') then
+        write (u) phic
+     else
+        write (u) q
      endif
      close (u)
      stop

@@ -1,6 +1,6 @@
 """Write the golden fixtures from the reference's own compiled hot path.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [gravity]
 
 Runs oracle/_ref/evref (the reference Fortran, built in place from the
 reference sources) on small seeded states and stores inputs and outputs as
@@ -124,5 +124,43 @@ def main():
     print(f"wrote {len(index)} fixtures")
 
 
+def gravity():
+    """multigreen and gravintegration (gravitation.f90:55-89,172-254) of the
+    reference on seeded fields, for grids with even, odd and power-of-two
+    sizes (grid spacings differ per axis).  The reference's full potential
+    needs FFTW-2, which this image lacks; these two routines do not."""
+    fx = {}
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    for nx, ny, nz in ((8, 6, 4), (16, 16, 16), (5, 7, 3), (12, 10, 9)):
+        p = HydroParams(nx=nx, ny=ny, nz=nz, gravity=True, atomic=True, xmax=2.0, ymax=1.5)
+        rng = np.random.default_rng(1000 * nx + 10 * ny + nz)
+        a = rng.standard_normal((nz, ny, nx)) + 1j * rng.standard_normal((nz, ny, nx))
+        name = f"green_{nx}x{ny}x{nz}"
+        fx[name + "/in"] = a
+        fx[name + "/out"] = O.ref_green(p, a)
+        index[name] = dict(kind="green", params=p.__dict__)
+        q = np.zeros((nz, ny, nx, 6))
+        q[..., 0] = 1.0 + 0.3 * rng.random((nz, ny, nx))
+        q[..., 1:4] = rng.standard_normal((nz, ny, nx, 3))
+        q[..., 4] = 5.0 + rng.random((nz, ny, nx))
+        q[..., 5] = 1.0
+        phi = rng.standard_normal((nz, ny, nx))
+        dt = 0.0123
+        name = f"gravint_{nx}x{ny}x{nz}"
+        fx[name + "/q0"] = q
+        fx[name + "/phi"] = phi
+        fx[name + "/q"] = O.ref_gravint(p, q, phi, dt)
+        index[name] = dict(kind="gravint", params=p.__dict__, dt=dt)
+    np.savez_compressed(os.path.join(HERE, "grav_fixtures.npz"), **fx)
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+    print(f"wrote {len(fx)} gravity arrays")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["gravity"]:
+        gravity()
+    else:
+        main()
+        gravity()
