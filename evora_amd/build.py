@@ -2,10 +2,11 @@
 
     python -m evora_amd.build          # builds evora_amd/libevora_hip.so
 
-Every translation unit is compiled for gfx950 only.  The staged and fused
-kernels are compiled without floating-point contraction so that their FP64
-results follow the reference's operation order; packed FP32 instructions are
-disabled for all device code.
+Every translation unit is compiled for gfx950 only.  The staged kernels, the
+exact builds of the fused and conduction kernels, and the gravity kernels are
+compiled without floating-point contraction so that their FP64 results follow
+the reference's operation order; packed FP32 instructions are disabled for
+all device code.  The library links hipFFT (ROCm) for the Poisson solve.
 """
 from __future__ import annotations
 
@@ -35,6 +36,7 @@ UNITS = [
     ("fused64x.o", "fused.hip", ["-DEVH_FUSED_PREC=64", "-DEVH_FUSED_EXACT=1", "-ffp-contract=off"]),
     ("conduction.o", "conduction.hip", []),
     ("conductionx.o", "conduction.hip", ["-DEVH_COND_EXACT=1", "-ffp-contract=off"]),
+    ("gravity.o", "gravity.hip", ["-ffp-contract=off"]),
     ("batch64.o", "batch.hip", ["-DEVH_FUSED_PREC=64"]),
     ("batch32.o", "batch.hip", ["-DEVH_FUSED_PREC=32"]),
 ]
@@ -62,7 +64,7 @@ def build_library(verbose: bool = False) -> str:
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
         list(ex.map(_run, cmds))
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lhipfft"])
     os.replace(tmp, OUT)
     shutil.rmtree(objdir, ignore_errors=True)
     return OUT

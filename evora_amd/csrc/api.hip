@@ -141,21 +141,28 @@ int grid_of(long long work) {
 bool mixed(const evh_ctx *h) { return h->cfg.precision == EVH_FP32_MIXED; }
 
 void accumulate_timing(evh_ctx *h) {
-  float ms = 0.f;
-  if (h->tev_pending) {
-    if (hipEventSynchronize(h->tev[1]) == hipSuccess && hipEventElapsedTime(&ms, h->tev[0], h->tev[1]) == hipSuccess) {
-      h->kernel_ms += ms;
-      h->kernel_launches += 1;
+  for (int i = 0; i < kStages; i++) {
+    if (!h->tev_pending[i]) continue;
+    float ms = 0.f;
+    if (hipEventSynchronize(h->tev[i][1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, h->tev[i][0], h->tev[i][1]) == hipSuccess) {
+      h->stage_ms[i] += ms;
+      h->stage_launches[i] += 1;
     }
-    h->tev_pending = false;
+    h->tev_pending[i] = false;
   }
-  if (h->tcv_pending) {
-    if (hipEventSynchronize(h->tcv[1]) == hipSuccess && hipEventElapsedTime(&ms, h->tcv[0], h->tcv[1]) == hipSuccess) {
-      h->cond_ms += ms;
-      h->cond_launches += 1;
-    }
-    h->tcv_pending = false;
-  }
+}
+
+// events around one launch of a stage, on the context's stream
+hipError_t timer_begin(evh_ctx *h, int stage) {
+  accumulate_timing(h);
+  return h->timing ? hipEventRecord(h->tev[stage][0], h->stream) : hipSuccess;
+}
+hipError_t timer_end(evh_ctx *h, int stage) {
+  if (!h->timing) return hipSuccess;
+  const hipError_t e = hipEventRecord(h->tev[stage][1], h->stream);
+  if (e == hipSuccess) h->tev_pending[stage] = true;
+  return e;
 }
 
 // Sync and surface device-side errors.  Errors stay sticky until read here.
@@ -202,6 +209,10 @@ int validate(const evh_config *c) {
   // unassigned dtempdx, conduction.f90:134-141), no dual energy (thermalstep
   // evaluates pressure on q a second time, timestep.f90:87), 3-D grids
   if (c->conduction && (!c->simplekappa || c->dualenergy || !fused_supported(*c) || !(c->temp0 > 0.0)))
+    return EVH_ERR_CONFIG;
+  // gravity: 3-D grids (the FFT Poisson solve of gravitation.f90 on one rank)
+  if (c->gravity && (!fused_supported(*c) || !(c->h2 > 0.0) || c->nyqx < 1 || c->nyqy < 1 || c->nyqz < 1 ||
+                     (c->supercomoving && !(c->fb > 0.0))))
     return EVH_ERR_CONFIG;
   return EVH_OK;
 }
@@ -270,10 +281,9 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
   if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e);
   for (auto &ev : h->events)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
-  for (auto &ev : h->tev)
-    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
-  for (auto &ev : h->tcv)
-    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
+  for (auto &pair : h->tev)
+    for (auto &ev : pair)
+      if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e);
   const long long n = h->grid.n();
   for (int b = 0; b < 2; b++) {
     if ((e = hipMalloc(&h->qbuf[b], h->qelem * 6 * n)) != hipSuccess) return bail(e);
@@ -283,6 +293,7 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
     if ((e = hipMalloc(&h->tbuf, h->qelem * n)) != hipSuccess) return bail(e);
     h->dev_bytes += (long long)h->qelem * n;
   }
+  if (h->cfg.gravity && (e = gravity_init(h)) != hipSuccess) return bail(e);
   if ((e = h->xfer.init(n)) != hipSuccess) return bail(e);
   if ((e = hipMalloc(&h->st, sizeof(Status))) != hipSuccess) return bail(e);
   if ((e = hipHostMalloc(&h->st_host, sizeof(Status))) != hipSuccess) return bail(e);
@@ -299,6 +310,7 @@ int evh_destroy(evh_ctx *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (auto p : h->qbuf) if (p) hipFree(p);
   if (h->tbuf) hipFree(h->tbuf);
+  gravity_release(h);
   if (h->w) hipFree(h->w);
   if (h->evo) hipFree(h->evo);
   if (h->s) hipFree(h->s);
@@ -307,8 +319,9 @@ int evh_destroy(evh_ctx *h) {
   if (h->st_host) hipHostFree(h->st_host);
   if (h->partials) hipFree(h->partials);
   for (auto ev : h->events) if (ev) hipEventDestroy(ev);
-  for (auto ev : h->tev) if (ev) hipEventDestroy(ev);
-  for (auto ev : h->tcv) if (ev) hipEventDestroy(ev);
+  for (auto &pair : h->tev)
+    for (auto ev : pair)
+      if (ev) hipEventDestroy(ev);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return EVH_OK;
@@ -437,13 +450,9 @@ int evh_sweep(evh_ctx *h, double dt) {
   h->evo_valid = false;
   if (h->mode == EVH_MODE_STAGED) {
     if (!h->bound_done) return fail(h, EVH_ERR_STATE, "sweep before bound");
-    accumulate_timing(h);
-    if (h->timing) HIP_TRY(hipEventRecord(h->tev[0], h->stream));
+    HIP_TRY(timer_begin(h, EVH_STAGE_SWEEP));
     HIP_TRY(launch_sweep(h, dt));
-    if (h->timing) {
-      HIP_TRY(hipEventRecord(h->tev[1], h->stream));
-      h->tev_pending = true;
-    }
+    HIP_TRY(timer_end(h, EVH_STAGE_SWEEP));
     h->w_valid = false;
     h->bound_done = false;
     h->tv_valid = false;
@@ -451,14 +460,10 @@ int evh_sweep(evh_ctx *h, double dt) {
     return check_status(h);
   }
   if (!h->v_valid) return fail(h, EVH_ERR_STATE, "sweep before primitive");
-  accumulate_timing(h);
   HIP_TRY(launch_reset_status(h, false));
-  if (h->timing) HIP_TRY(hipEventRecord(h->tev[0], h->stream));
+  HIP_TRY(timer_begin(h, EVH_STAGE_SWEEP));
   HIP_TRY(launch_fused_step(h, dt));
-  if (h->timing) {
-    HIP_TRY(hipEventRecord(h->tev[1], h->stream));
-    h->tev_pending = true;
-  }
+  HIP_TRY(timer_end(h, EVH_STAGE_SWEEP));
   h->cur = 1 - h->cur;
   h->v_valid = true;
   h->tv_valid = false;
@@ -496,18 +501,52 @@ int evh_thermalconduction(evh_ctx *h, double dt) {
   if (!h->cfg.conduction) return fail(h, EVH_ERR_STATE, "thermalconduction without conduction");
   if (!h->t_valid) HIP_TRY(launch_temp(h));
   HIP_TRY(launch_reset_status(h, false));
-  accumulate_timing(h);
-  if (h->timing) HIP_TRY(hipEventRecord(h->tcv[0], h->stream));
+  HIP_TRY(timer_begin(h, EVH_STAGE_CONDUCTION));
   HIP_TRY(launch_conduction(h, dt));
-  if (h->timing) {
-    HIP_TRY(hipEventRecord(h->tcv[1], h->stream));
-    h->tcv_pending = true;
-  }
+  HIP_TRY(timer_end(h, EVH_STAGE_CONDUCTION));
   h->v_valid = true;
   h->tv_valid = true;
   h->t_valid = false;  // E changed
   h->w_valid = false;
   h->bound_done = false;
+  return EVH_OK;
+}
+
+// gravitation.f90:26-52: the potential of the current q (phi stays on the device)
+int evh_gravpotential(evh_ctx *h, double cosmo) {
+  if (!h) return EVH_ERR_STATE;
+  if (!h->cfg.gravity) return fail(h, EVH_ERR_STATE, "gravpotential without gravity");
+  h->cosmo = cosmo;
+  HIP_TRY(timer_begin(h, EVH_STAGE_GRAVPOTENTIAL));
+  HIP_TRY(launch_gravpotential(h, cosmo));
+  HIP_TRY(timer_end(h, EVH_STAGE_GRAVPOTENTIAL));
+  h->phi_valid = true;
+  return EVH_OK;
+}
+
+// gravitation.f90:172-254 with the last potential
+int evh_gravintegration(evh_ctx *h, double dt) {
+  if (!h) return EVH_ERR_STATE;
+  if (!h->cfg.gravity) return fail(h, EVH_ERR_STATE, "gravintegration without gravity");
+  if (!h->phi_valid) return fail(h, EVH_ERR_STATE, "gravintegration before gravpotential");
+  HIP_TRY(launch_reset_status(h, false));
+  HIP_TRY(timer_begin(h, EVH_STAGE_GRAVINTEGRATION));
+  HIP_TRY(launch_gravintegration(h, dt));
+  HIP_TRY(timer_end(h, EVH_STAGE_GRAVINTEGRATION));
+  h->v_valid = true;                // the kernel formed the CFL minimum of the new state
+  h->tv_valid = false;
+  h->t_valid = h->tbuf != nullptr;  // and T, with conduction on
+  h->w_valid = false;
+  h->bound_done = false;
+  return EVH_OK;
+}
+
+int evh_download_phi(evh_ctx *h, double *phi) {
+  if (!h || !phi) return EVH_ERR_STATE;
+  if (!h->phi_valid) return fail(h, EVH_ERR_STATE, "no potential yet");
+  int r = check_status(h);
+  if (r) return r;
+  HIP_TRY(hipMemcpy(phi, h->phi, sizeof(double) * h->grid.n(), hipMemcpyDeviceToHost));
   return EVH_OK;
 }
 
@@ -526,6 +565,10 @@ int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go) {
   *t = *t + *dt;
   if (h->cfg.secondorder && (r = evh_compevo(h, *dt))) return r;
   if ((r = evh_sweep(h, *dt))) return r;
+  if (h->cfg.gravity) {  // main.f90:157-164
+    if ((r = evh_gravpotential(h, h->cosmo))) return r;
+    if ((r = evh_gravintegration(h, *dt))) return r;
+  }
   if (h->cfg.conduction) return evh_thermalconduction(h, *dt);  // main.f90:170-175
   return EVH_OK;
 }
@@ -661,8 +704,10 @@ int evh_set_timing(evh_ctx *h, int on) {
   int r = check_status(h);
   if (r) return r;
   h->timing = on != 0;
-  h->kernel_ms = h->cond_ms = 0.0;
-  h->kernel_launches = h->cond_launches = 0;
+  for (int i = 0; i < kStages; i++) {
+    h->stage_ms[i] = 0.0;
+    h->stage_launches[i] = 0;
+  }
   return EVH_OK;
 }
 
@@ -670,8 +715,8 @@ int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches) {
   if (!h || !total_ms || !launches) return EVH_ERR_STATE;
   int r = check_status(h);
   if (r) return r;
-  *total_ms = h->kernel_ms;
-  *launches = h->kernel_launches;
+  *total_ms = h->stage_ms[EVH_STAGE_SWEEP];
+  *launches = h->stage_launches[EVH_STAGE_SWEEP];
   return EVH_OK;
 }
 
@@ -679,8 +724,18 @@ int evh_conduction_stats(evh_ctx *h, double *total_ms, long long *launches) {
   if (!h || !total_ms || !launches) return EVH_ERR_STATE;
   int r = check_status(h);
   if (r) return r;
-  *total_ms = h->cond_ms;
-  *launches = h->cond_launches;
+  *total_ms = h->stage_ms[EVH_STAGE_CONDUCTION];
+  *launches = h->stage_launches[EVH_STAGE_CONDUCTION];
+  return EVH_OK;
+}
+
+int evh_stage_stats(evh_ctx *h, int stage, double *total_ms, long long *launches) {
+  if (!h || !total_ms || !launches) return EVH_ERR_STATE;
+  if (stage < 0 || stage >= kStages) return fail(h, EVH_ERR_CONFIG, "no such stage");
+  int r = check_status(h);
+  if (r) return r;
+  *total_ms = h->stage_ms[stage];
+  *launches = h->stage_launches[stage];
   return EVH_OK;
 }
 

@@ -25,6 +25,8 @@ struct Grid {
 
 }  // namespace evh
 
+constexpr int kStages = 4;  // EVH_STAGE_SWEEP .. EVH_STAGE_GRAVINTEGRATION
+
 struct evh_ctx {
   evh_config cfg;
   evh::KParams kp;
@@ -46,6 +48,17 @@ struct evh_ctx {
   // by the fused sweep, read by the conduction pass; storage precision
   void *tbuf = nullptr;
   bool t_valid = false;
+
+  // self-gravity (gravity on): potential, its half spectrum (real-to-complex
+  // transform along x), the sin^2 tables of the Green's function (x, y, z,
+  // full ranges), partial sums of rho and their mean, and the FFT plans
+  double *phi = nullptr;
+  void *spec = nullptr;
+  double *sin2 = nullptr;
+  double *rho_part = nullptr;
+  void *plan_fwd = nullptr, *plan_inv = nullptr;  // hipfftHandle
+  bool phi_valid = false;
+  double cosmo = 1.0;  // expansion factor of the last evh_gravpotential
 
   // staged path arrays (allocated on first use)
   double *w = nullptr, *evo = nullptr, *s = nullptr;
@@ -76,14 +89,11 @@ struct evh_ctx {
 
   // kernel timing (evh_set_timing)
   bool timing = false;
-  hipEvent_t tev[2] = {nullptr, nullptr};
-  bool tev_pending = false;
-  double kernel_ms = 0.0;
-  long long kernel_launches = 0;
-  hipEvent_t tcv[2] = {nullptr, nullptr};  // the same for conduction passes
-  bool tcv_pending = false;
-  double cond_ms = 0.0;
-  long long cond_launches = 0;
+  // per stage (EVH_STAGE_*): start/end events of the last launch, totals
+  hipEvent_t tev[kStages][2] = {};
+  bool tev_pending[kStages] = {};
+  double stage_ms[kStages] = {};
+  long long stage_launches[kStages] = {};
 };
 
 namespace evh {
@@ -109,6 +119,12 @@ hipError_t launch_thermal_min_exact(evh_ctx *h);        // the same with the ref
 hipError_t launch_temp_exact(evh_ctx *h);
 hipError_t launch_conduction_exact(evh_ctx *h, double dt);
 bool fused_supported(const evh_config &c);
+
+// gravity (gravity.hip)
+hipError_t gravity_init(evh_ctx *h);     // buffers, tables and FFT plans
+void gravity_release(evh_ctx *h);
+hipError_t launch_gravpotential(evh_ctx *h, double cosmo);  // phi of cur q
+hipError_t launch_gravintegration(evh_ctx *h, double dt);   // cur q in place with phi, limits (and T) of the result
 
 // KParams of a configuration (api.hip)
 KParams make_kparams(const evh_config &c);

@@ -32,6 +32,10 @@ module evora_hip
      real(c_double) :: volume, mach_fp64
      integer(c_int) :: conduction, simplekappa
      real(c_double) :: c_cond, temp0, kappa0
+     integer(c_int) :: gravity, supercomoving
+     real(c_double) :: grav, fb, h2, kfx2, kfy2, kfz2
+     integer(c_int) :: nyqx, nyqy, nyqz
+     real(c_double) :: dx2inv, dy2inv, dz2inv
   end type evh_config
 
   interface
@@ -52,6 +56,18 @@ module evora_hip
        type(c_ptr), value :: h
        real(c_double), value :: dt
      end function evh_thermalconduction
+
+     integer(c_int) function evh_gravpotential(h, cosmo) bind(c, name='evh_gravpotential')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: cosmo
+     end function evh_gravpotential
+
+     integer(c_int) function evh_gravintegration(h, dt) bind(c, name='evh_gravintegration')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: dt
+     end function evh_gravintegration
 
      integer(c_int) function evh_destroy(h) bind(c, name='evh_destroy')
        import :: c_int, c_ptr

@@ -32,10 +32,12 @@
 ! (tend - t <= dt, the test of timestep.f90:163).  w, evo and s are never
 ! filled on the host; pure hydro never reads them there.  Thermal
 ! conduction (thermalstep, thermalconduction with simplekappa, without dual
-! energy) runs on the device too.  Physics that update q on the host between
-! steps (gravity, chemistry, floors, supercomoving expansion), conduction
-! with the full kappa or with dual energy, and multi-rank runs are refused
-! on the GPU backend.
+! energy) and self-gravity on 3-D grids (gravpotential, gravintegration; the
+! potential stays on the device) run on the device too.  Physics that
+! update q on the host between steps (chemistry, floors), supercomoving
+! runs (whose host-side time and snapshot rules this dispatch does not
+! mirror), conduction with the full kappa or with dual energy, and
+! multi-rank runs are refused on the GPU backend.
 !-----------------------------------------------------------------------------
 module hydro_dispatch_state
   use iso_c_binding
@@ -129,9 +131,13 @@ contains
   subroutine hip_create()
     use global
     type(evh_config) :: cfg
-    if (gravity .or. chemistry .or. supercomoving .or. tempfloor .or. jeansfloor &
+    if (chemistry .or. supercomoving .or. tempfloor .or. jeansfloor &
          .or. cooldt .or. noncie .or. parallel .or. .not. hydro) then
-       print *, 'ERROR: EVORA_BACKEND=hip runs hydro (and thermal conduction) on one rank only'
+       print *, 'ERROR: EVORA_BACKEND=hip runs hydro (with thermal conduction and gravity) on one rank only'
+       call cleanstop
+    endif
+    if (gravity .and. (onedimension .or. twodimension)) then
+       print *, 'ERROR: EVORA_BACKEND=hip runs gravity on 3-D grids only'
        call cleanstop
     endif
     if (conduction .and. (.not. simplekappa .or. dualenergy)) then
@@ -154,6 +160,12 @@ contains
     cfg%conduction = merge(1, 0, conduction)
     cfg%simplekappa = merge(1, 0, simplekappa)
     cfg%c_cond = c_cond ; cfg%temp0 = temp0 ; cfg%kappa0 = kappa0
+    cfg%gravity = merge(1, 0, gravity)
+    cfg%supercomoving = merge(1, 0, supercomoving)
+    cfg%grav = grav ; cfg%fb = fb ; cfg%h2 = h2
+    cfg%kfx2 = kfx2 ; cfg%kfy2 = kfy2 ; cfg%kfz2 = kfz2
+    cfg%nyqx = nyqx ; cfg%nyqy = nyqy ; cfg%nyqz = nyqz
+    cfg%dx2inv = dx2inv ; cfg%dy2inv = dy2inv ; cfg%dz2inv = dz2inv
     call hip_check(evh_create(cfg, ctx))
     ! 3-D grids start in the fused form; the exact form on request (FP64)
     if (exact .and. precision .eq. EVH_FP64 .and. .not. onedimension .and. .not. twodimension) &
@@ -261,9 +273,46 @@ subroutine sweep(w, q, evo, s)
      return
   endif
   call hip_check(evh_sweep(ctx, dt))
-  ! with conduction the step ends in thermalconduction, which downloads
-  if (.not. conduction) call download_if_read(q)
+  ! with gravity or conduction the step ends in gravintegration or
+  ! thermalconduction, which download
+  if (.not. (conduction .or. gravity)) call download_if_read(q)
 end subroutine sweep
+
+! gravitation.f90:26-52 (main.f90:159): on the device the potential stays
+! there for gravintegration; the host phi is not filled (main.f90 does not
+! read it)
+subroutine gravpotential(q, phi)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), intent(in), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+  complex(8), intent(out), dimension(lx:mx,ly:my,lz:mz) :: phi
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_gravpotential(q, phi)
+     return
+  endif
+  call hip_check(evh_gravpotential(ctx, cosmo))
+end subroutine gravpotential
+
+! gravitation.f90:172-254 (main.f90:163)
+subroutine gravintegration(q, phi, phibuf2, phibuf4)
+  use global
+  use hydro_dispatch_state
+  implicit none
+  real(8), intent(inout), dimension(nq,lx:mx,ly:my,lz:mz) :: q
+  complex(8), intent(in), dimension(lx:mx,ly:my,lz:mz) :: phi
+  real(8), intent(in), dimension(lx:mx,ly:my) :: phibuf2, phibuf4
+
+  call decide_backend()
+  if (backend .eq. 0) then
+     call evcpu_gravintegration(q, phi, phibuf2, phibuf4)
+     return
+  endif
+  call hip_check(evh_gravintegration(ctx, dt))
+  if (.not. conduction) call download_if_read(q)
+end subroutine gravintegration
 
 ! snapshot.f90:159-193.  Before the state is on the device (the call of
 ! main.f90:100) and on the CPU backend this is the reference routine; on the

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libevora_hi
 
 FP64, FP32_MIXED = 0, 1
 MODE_FUSED, MODE_STAGED, MODE_FUSED_EXACT = 0, 1, 2
+STAGE_SWEEP, STAGE_CONDUCTION, STAGE_GRAVPOTENTIAL, STAGE_GRAVINTEGRATION = 0, 1, 2, 3
 
 ERR_NAMES = {
     1: "negative pressure", 2: "riemann unphysical pressure", 3: "hll", 4: "hllc", 5: "dt",
@@ -37,7 +38,8 @@ EXPORTS = (
     "evh_thermalstep", "evh_thermalconduction",
     "evh_step", "evh_run", "evh_sums", "evh_download_w", "evh_download_evo_s",
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
-    "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_conduction_stats",
+    "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_conduction_stats", "evh_stage_stats",
+    "evh_gravpotential", "evh_gravintegration", "evh_download_phi",
     "evh_device_bytes",
     "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
     "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
@@ -60,6 +62,11 @@ class EvhConfig(C.Structure):
         ("mach_fp64", C.c_double),
         ("conduction", C.c_int), ("simplekappa", C.c_int),
         ("c_cond", C.c_double), ("temp0", C.c_double), ("kappa0", C.c_double),
+        ("gravity", C.c_int), ("supercomoving", C.c_int),
+        ("grav", C.c_double), ("fb", C.c_double), ("h2", C.c_double),
+        ("kfx2", C.c_double), ("kfy2", C.c_double), ("kfz2", C.c_double),
+        ("nyqx", C.c_int), ("nyqy", C.c_int), ("nyqz", C.c_int),
+        ("dx2inv", C.c_double), ("dy2inv", C.c_double), ("dz2inv", C.c_double),
     ]
 
 
@@ -102,6 +109,10 @@ def load_library():
             "evh_get_tuning": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "evh_kernel_stats": [P, D, C.POINTER(C.c_longlong)],
             "evh_conduction_stats": [P, D, C.POINTER(C.c_longlong)],
+            "evh_stage_stats": [P, C.c_int, D, C.POINTER(C.c_longlong)],
+            "evh_gravpotential": [P, C.c_double],
+            "evh_gravintegration": [P, C.c_double],
+            "evh_download_phi": [P, D],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -225,6 +236,20 @@ class Hydro:
         """conduction.f90:26-123 (with the primitive and bound before it)"""
         self._check(self.L.evh_thermalconduction(self.h, dt))
 
+    def gravpotential(self, cosmo: float = 1.0):
+        """gravitation.f90:26-52: the potential of the current state (on the device)."""
+        self._check(self.L.evh_gravpotential(self.h, cosmo))
+
+    def gravintegration(self, dt: float):
+        """gravitation.f90:172-254 with the last potential."""
+        self._check(self.L.evh_gravintegration(self.h, dt))
+
+    def download_phi(self) -> np.ndarray:
+        """The last potential, shape (nz, ny, nx) == phi(nx, ny, nz)."""
+        phi = np.empty((self.d.nz, self.d.ny, self.d.nx))
+        self._check(self.L.evh_download_phi(self.h, _dptr(phi)))
+        return phi
+
     def step(self, t: float, tend: float):
         """One pass of main.f90:109-155; returns (t, dt, go)."""
         tt, dt, go = C.c_double(t), C.c_double(0.0), C.c_int(1)
@@ -289,6 +314,12 @@ class Hydro:
         """(total ms, launches) of the conduction kernel since set_timing(True)."""
         ms, n = C.c_double(), C.c_longlong()
         self._check(self.L.evh_conduction_stats(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def stage_stats(self, stage: int):
+        """(total ms, launches) of a stage (STAGE_*) since set_timing(True)."""
+        ms, n = C.c_double(), C.c_longlong()
+        self._check(self.L.evh_stage_stats(self.h, stage, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
     def device_bytes(self) -> int:

@@ -59,6 +59,12 @@ typedef struct evh_config {
    * simplekappa, without dual energy, on 3-D grids */
   int conduction, simplekappa;
   double c_cond, temp0, kappa0;
+  /* self-gravity (parameter.f90:235-241,254-263): 3-D, one rank; the
+   * potential is periodic whatever the boundtype, as in the reference */
+  int gravity, supercomoving;
+  double grav, fb, h2, kfx2, kfy2, kfz2;
+  int nyqx, nyqy, nyqz;
+  double dx2inv, dy2inv, dz2inv;
 } evh_config;
 
 typedef struct evh_ctx evh_ctx;
@@ -95,10 +101,19 @@ int evh_thermalconduction(evh_ctx *h, double dt); /* thermalconduction(q,w) cond
                                                     with the primitive and bound main.f90:171-172
                                                     calls before it; simplekappa only        */
 
-/* One pass of main.f90:109-155 on the device (with thermalstep and the
- * conduction of main.f90:121,170-175 when conduction is on), with
- * updatetime (timestep.f90:163-169): t, dt, go in/out.  evh_run repeats it
- * nsteps times (or until go turns 0), syncing only for dt. */
+/* self-gravity (gravity on in the config): */
+int evh_gravpotential(evh_ctx *h, double cosmo);  /* gravpotential(q,phi) gravitation.f90:26-52: the potential of
+                                                    the current q, kept on the device; cosmo is the expansion
+                                                    factor of the supercomoving source (ignored otherwise) */
+int evh_gravintegration(evh_ctx *h, double dt);   /* gravintegration(q,phi,..) gravitation.f90:172-254 */
+int evh_download_phi(evh_ctx *h, double *phi);    /* the potential phi(nx,ny,nz) of the last evh_gravpotential */
+
+/* One pass of main.f90:109-175 on the device (with thermalstep and the
+ * conduction of main.f90:121,170-175 when conduction is on, and the
+ * gravity of main.f90:157-164 when gravity is on, with the expansion factor
+ * of the last evh_gravpotential call, 1 at first), with updatetime
+ * (timestep.f90:163-169): t, dt, go in/out.  evh_run repeats it nsteps
+ * times (or until go turns 0), syncing only for dt. */
 int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go);
 int evh_run(evh_ctx *h, int nsteps, double *t, double tend, double *dt, int *done);
 
@@ -126,13 +141,17 @@ int evh_set_tuning(evh_ctx *h, int tile_ty, int wpe_hi, int zchunk);
 int evh_get_tuning(evh_ctx *h, int *tile_ty, int *wpe_hi, int *zchunk);
 
 /* Kernel timing (no counterpart in the reference; for benchmarks): when on,
- * every fused or staged sweep launch and every conduction-kernel launch is
- * bracketed by events on the context's stream and its duration is added to a
- * running total at the next sync. */
+ * every fused or staged sweep launch, conduction-kernel launch and gravity
+ * stage is bracketed by events on the context's stream and its duration is
+ * added to a running total at the next sync. */
 int evh_set_timing(evh_ctx *h, int on);
 int evh_kernel_stats(evh_ctx *h, double *total_ms, long long *launches);
 /* the same for the conduction kernel of evh_thermalconduction */
 int evh_conduction_stats(evh_ctx *h, double *total_ms, long long *launches);
+/* the same for any stage: the sweep, the conduction kernel, the potential
+ * (mean, source, both FFTs and the Green's function) or the force update */
+enum { EVH_STAGE_SWEEP = 0, EVH_STAGE_CONDUCTION = 1, EVH_STAGE_GRAVPOTENTIAL = 2, EVH_STAGE_GRAVINTEGRATION = 3 };
+int evh_stage_stats(evh_ctx *h, int stage, double *total_ms, long long *launches);
 
 /* Bytes of device memory the context holds. */
 long long evh_device_bytes(evh_ctx *h);

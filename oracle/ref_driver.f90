@@ -10,8 +10,11 @@
 !               (parallel.f90:33-64)
 !   arrays      q, w, evo, s allocated and zeroed as main.f90:72-77
 !   loop        primitive, bound, dt=1d99, hydrostep, [thermalstep],
-!               updatetime, compevo, sweep, [primitive, bound,
-!               thermalconduction]   (main.f90:109-175; [] with conduction)
+!               updatetime, compevo, sweep, {gravpotential,
+!               gravintegration}, [primitive, bound, thermalconduction]
+!               (main.f90:109-175; [] with conduction, {} with gravity: the
+!               reference's potential needs FFTW-2, absent here, so runs
+!               with gravity are for the GPU drop-in, EVORA_BACKEND=hip)
 !
 ! usage: evref <parafile> <mode> <in.bin> <out.bin> [nsteps]
 !   mode run    : in = q(nq,nx,ny,nz) f64; nsteps passes; out = q
@@ -135,6 +138,17 @@ program evref
   read (u) q
   close (u)
 
+  if (gravity) then
+     call get_environment_variable('EVORA_BACKEND', arg, l, st)
+     if (st .ne. 0 .or. arg(1:max(l, 1)) .ne. 'hip') then
+        print *, 'ERROR: the reference potential needs FFTW-2 (not in this image); run gravity with EVORA_BACKEND=hip'
+        stop
+     endif
+     allocate(phic(lx:mx,ly:my,lz:mz), phibuf2(lx:mx,ly:my), phibuf4(lx:mx,ly:my))
+     phibuf2 = 0.0
+     phibuf4 = 0.0
+  endif
+
   go = .true.
   dump = .false.
   k4 = 1
@@ -165,6 +179,10 @@ program evref
         write (u) s
      endif
      call sweep(w, q, evo, s)
+     if (gravity) then                          ! main.f90:157-164
+        call gravpotential(q, phic)
+        call gravintegration(q, phic, phibuf2, phibuf4)
+     endif
      if (conduction) then                       ! main.f90:170-175
         call primitive(w, q)
         call bound(w)

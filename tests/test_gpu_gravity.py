@@ -1,0 +1,177 @@
+"""Self-gravity on the GPU (evh_gravpotential, evh_gravintegration) against
+the oracle, whose multigreen and gravintegration are bit-identical to the
+reference's and whose transforms match numpy's (tests/test_oracle_gravity.py).
+
+Bars: the potential within 1e-12 of the oracle's (relative to max |phi|; the
+FFTs differ from the direct DFT sums by rounding only); the fast fused
+step as for hydro (relative L-infinity <= 1e-12 after one step, relative
+L1 <= 1e-10 after 10); the exact form within 1e-12 after 10 steps (the
+FFT rounding rules out bit equality); FP32-mixed relative L1 of rho and p
+<= 1e-5.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l1, rel_linf
+from evora_amd import hydro, ics
+from evora_amd.params import HydroParams
+from test_gpu_dropin import run_dropin
+
+pytestmark = pytest.mark.gpu
+
+
+def _potential(d, q, cosmo=1.0, precision=hydro.FP64):
+    with hydro.Hydro(d, precision=precision) as h:
+        h.upload_q(q)
+        h.gravpotential(cosmo)
+        return h.download_phi()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (12, 10, 9), (24, 20, 18), (7, 9, 5)])
+def test_potential_matches_oracle(shape):
+    nx, ny, nz = shape
+    p = ics.jeans_params(nx, ny=ny, nz=nz, xmax=nx / 16, ymax=ny / 16, zmax=nz / 16)
+    d = p.derive()
+    q = ics.jeans(p)
+    phi = _potential(d, q)
+    ref = O.Oracle(d).gravpotential(q)
+    assert rel_linf(phi, ref) <= 1e-12
+
+
+def test_potential_supercomoving_source():
+    p = HydroParams(nx=16, ny=12, nz=10, gravity=True, supercomoving=True, fb=0.17, atomic=True)
+    d = p.derive()
+    rng = np.random.default_rng(4)
+    q = ics.random_smooth(p, seed=3)
+    q[..., 0] = 0.1 + rng.random(q.shape[:3])
+    cosmo = 0.37
+    assert rel_linf(_potential(d, q, cosmo), O.Oracle(d).gravpotential(q, cosmo)) <= 1e-12
+
+
+def test_potential_fp32_mixed_source():
+    """The mixed form takes rho from its FP32 state; the transforms stay FP64."""
+    p = ics.jeans_params(16)
+    d = p.derive()
+    q = ics.jeans(p)
+    q32 = q.astype(np.float32).astype(np.float64)
+    assert rel_linf(_potential(d, q, precision=hydro.FP32_MIXED), O.Oracle(d).gravpotential(q32)) <= 1e-12
+
+
+@pytest.mark.parametrize("mode", [hydro.MODE_FUSED, hydro.MODE_FUSED_EXACT])
+def test_gravity_one_step(mode):
+    p = ics.jeans_params(32)
+    d = p.derive()
+    q0 = ics.jeans(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 1)
+    with hydro.Hydro(d, mode=mode) as h:
+        h.upload_q(q0)
+        h.run(1)
+        q = h.download_q()
+    for c in range(6):
+        assert rel_linf(q[..., c], qo[..., c]) <= 1e-12, c
+
+
+@pytest.mark.parametrize("mode,tol", [(hydro.MODE_FUSED, 1e-10), (hydro.MODE_FUSED_EXACT, 1e-12)])
+def test_gravity_ten_steps(mode, tol):
+    p = ics.jeans_params(24, nz=20, zmax=20 / 24)
+    d = p.derive()
+    q0 = ics.jeans(p)
+    qo = q0.copy()
+    to, dto, _ = O.Oracle(d).run(qo, 10)
+    with hydro.Hydro(d, mode=mode) as h:
+        h.upload_q(q0)
+        t, dt, n = h.run(10)
+        q = h.download_q()
+    assert n == 10
+    assert t == pytest.approx(to, rel=1e-12)
+    for c in range(6):
+        assert rel_l1(q[..., c], qo[..., c]) <= tol, c
+
+
+def test_gravity_mixed_precision():
+    p = ics.jeans_params(24)
+    d = p.derive()
+    q0 = ics.jeans(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 5)
+    with hydro.Hydro(d, precision=hydro.FP32_MIXED) as h:
+        h.upload_q(q0)
+        h.run(5)
+        q = h.download_q()
+    for c in (0, 4):
+        assert rel_l1(q[..., c], qo[..., c]) <= 1e-5, c
+
+
+def test_gravity_with_conduction():
+    """main.f90:157-175: sweep, then gravity, then conduction (T taken from
+    the state after the gravity update)."""
+    p = ics.conduction_params(16, boundtype=2, gravity=True)
+    d = p.derive()
+    d = dataclasses.replace(d, grav=5e4)  # strong enough to matter on this hot blob (Jeans length ~ 1 box)
+    q0 = ics.conduction_blob(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 3)
+    with hydro.Hydro(d) as h:
+        h.upload_q(q0)
+        h.run(3)
+        q = h.download_q()
+    qh = q0.copy()
+    O.Oracle(dataclasses.replace(d, gravity=0)).run(qh, 3)
+    assert rel_l1(qo[..., 1], qh[..., 1]) > 1e-6  # gravity changed the run
+    for c in range(6):
+        assert rel_l1(q[..., c], qo[..., c]) <= 1e-10, c
+
+
+def test_gravity_fullsize_conservation():
+    """256^3 Jeans box, 3 steps: mass conserved to roundoff, and total
+    momentum too (periodic hydro and a central-difference force of a
+    potential with an even Green's function both conserve it)."""
+    p = ics.jeans_params(256)
+    d = p.derive()
+    q0 = ics.jeans(p)
+    with hydro.Hydro(d) as h:
+        h.upload_q(q0)
+        s0 = h.sums()
+        h.run(3)
+        s1 = h.sums()
+        q = h.download_q()
+    assert s1[0] == pytest.approx(s0[0], rel=1e-13)
+    dm = q[..., 1:4].sum(axis=(0, 1, 2)) - q0[..., 1:4].sum(axis=(0, 1, 2))
+    assert np.abs(dm).max() <= 1e-13 * np.abs(q[..., 1:4]).sum()
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_gravity_dropin(exact):
+    """The Fortran drop-in with gravity on (gravpotential and gravintegration
+    dispatched to the GPU) against the oracle's run: the reference's own
+    potential cannot be built here (FFTW-2)."""
+    p = ics.jeans_params(20, tend=1e30)
+    q0 = ics.jeans(p)
+    env = {"EVORA_MAXSTEPS": "6"}
+    if exact:
+        env["EVORA_EXACT"] = "1"
+    qh, sh = run_dropin(p, q0, 1000, "hip", env)
+    assert len(sh) == 7  # 6 steps and the dt = 0 step that ends the run
+    qo = q0.copy()
+    O.Oracle(p.derive()).run(qo, 6)
+    for c in range(6):
+        assert rel_l1(qh[..., c], qo[..., c]) <= 1e-10, c
+
+
+def test_gravity_refusals():
+    with pytest.raises(hydro.EvoraError) as ei:
+        hydro.Hydro(HydroParams(nx=16, ny=16, nz=1, gravity=True).derive())
+    assert ei.value.code == 8
+    p = ics.jeans_params(8)
+    with hydro.Hydro(p.derive()) as h:
+        h.upload_q(ics.jeans(p))
+        with pytest.raises(hydro.EvoraError) as ei:
+            h.gravintegration(0.1)
+        assert ei.value.code == 10
+    with hydro.Hydro(ics.blast_params(8).derive()) as h:
+        with pytest.raises(hydro.EvoraError):
+            h.gravpotential()
