@@ -18,8 +18,10 @@ alongside, the roofline of the dominant kernel (HIP-event timing of each
 sweep launch on the library's stream), the FP64 fused step in its exact form
 (the reference's own arithmetic, bit-identical results), the FP32-mixed and
 batched forms, the step with thermal conduction on (whole steps against the
-hydro step alone, and the conduction kernel, on a 512^3 hot blob), and the
-reference's CPU hot path timed on the host's cores on a bounded sample.
+hydro step alone, and the conduction kernel, on a 512^3 hot blob), the step
+with self-gravity on (against the hydro step alone, with the potential and
+the force kernel timed, on a 512^3 Jeans-unstable box), and the reference's
+CPU hot path timed on the host's cores on a bounded sample.
 Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
 HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50 steps, the
 cosmological-like field 256^3 for 100 steps, and the batched ensemble) is
@@ -46,6 +48,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"fp64": 96.0, "fp32_mixed": 48.0}  # read + write 6 conserved values
 COND_BYTES_PER_CELL = {"fp64": 64.0, "fp32_mixed": 32.0}  # read 6 conserved values, write 2
+# gravintegration: read rho, momentum, E (5 values) and phi (FP64), write momentum and E (4 values)
+GRAV_BYTES_PER_CELL = {"fp64": 80.0, "fp32_mixed": 44.0}
 
 
 def log(msg):
@@ -119,6 +123,8 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
         wall = time.perf_counter() - t0
         kms, launches = h.kernel_stats()
         cms, claunches = h.conduction_stats()
+        gps, gpn = h.stage_stats(hydro.STAGE_GRAVPOTENTIAL)
+        gis, gin = h.stage_stats(hydro.STAGE_GRAVINTEGRATION)
         sums = h.sums()
         t_dn = time.perf_counter()
         q = h.download_q() if keep_q or time_xfer else None
@@ -131,6 +137,8 @@ def run_gpu(d, q0, steps, warmup, precision_name="fp64", tend=1e30, keep_q=False
         kernel_ms=kms / max(launches, 1),
         launches=launches,
         cond_ms=cms / max(claunches, 1),
+        gravpot_ms=gps / max(gpn, 1),
+        gravint_ms=gis / max(gin, 1),
         t_end=t,
         sums0=sums0,
         sums=sums,
@@ -170,6 +178,38 @@ def run_conduction(n, steps, warmup, fp64_only=False):
                          conduction_GBs=ach, conduction_hbm_frac=ach / HBM_PEAK_GBS)
         log(f"conduction {prec}: kernel {ms:.3f} ms ({ach:.0f} GB/s), step {r['ms_per_step']:.3f} ms "
             f"vs hydro alone {rh['ms_per_step']:.3f} ms")
+    return out
+
+
+def run_gravity(n, steps, warmup, fp64_only=False):
+    """The step with self-gravity on (hydro step, then gravpotential and
+    gravintegration, main.f90:157-164) on an n^3 Jeans-unstable periodic box,
+    FP64 fused (and FP32-mixed), against the hydro step alone on the same
+    state.  The potential (mean density, source, forward FFT, Green's
+    function, inverse FFT; hipFFT in FP64 for both forms) and the force
+    kernel are timed per launch with events on the library's stream; the
+    force kernel's algorithmic bytes are its compulsory ones (read rho,
+    momentum, E and phi, write momentum and E)."""
+    from evora_amd import ics
+
+    p = ics.jeans_params(n)
+    d = p.derive()
+    dh = ics.jeans_params(n, gravity=False).derive()
+    log(f"building {n}^3 Jeans initial condition")
+    q0 = ics.jeans(p)
+    out = {"workload": f"Jeans-unstable periodic box {n}^3 (rho 1e6 with a 10% box mode, Jeans length 0.5), "
+                       f"HLLC, minmod, CFL 0.4, 4 pi G = 8.38553911e-7"}
+    for prec in ("fp64",) if fp64_only else ("fp64", "fp32_mixed"):
+        r = run_gpu(d, q0, steps, warmup, prec)
+        rh = run_gpu(dh, q0, steps, warmup, prec)
+        nbytes = GRAV_BYTES_PER_CELL[prec]
+        ach = nbytes * d.ncells / (r["gravint_ms"] * 1e-3) / 1e9
+        out[prec] = dict(value=r["value"], ms_per_step=r["ms_per_step"], hydro_only_ms_per_step=rh["ms_per_step"],
+                         sweep_kernel_ms=r["kernel_ms"], gravpotential_ms=r["gravpot_ms"],
+                         gravintegration_kernel_ms=r["gravint_ms"], gravintegration_bytes_per_cell=nbytes,
+                         gravintegration_GBs=ach, gravintegration_hbm_frac=ach / HBM_PEAK_GBS)
+        log(f"gravity {prec}: potential {r['gravpot_ms']:.3f} ms, force {r['gravint_ms']:.3f} ms ({ach:.0f} GB/s), "
+            f"step {r['ms_per_step']:.3f} ms vs hydro alone {rh['ms_per_step']:.3f} ms")
     return out
 
 
@@ -374,6 +414,9 @@ def main():
     ap.add_argument("--no-conduction", action="store_true", help="skip the thermal-conduction form")
     ap.add_argument("--conduction-only", action="store_true",
                     help="run only the thermal-conduction form and print its JSON (for profiling)")
+    ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity form")
+    ap.add_argument("--gravity-only", action="store_true",
+                    help="run only the self-gravity form and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--batch-boxes", type=int, default=48)
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -390,6 +433,9 @@ def main():
 
     if args.conduction_only:
         print(json.dumps(run_conduction(args.n, args.steps, args.warmup, args.fp64_only)), flush=True)
+        return
+    if args.gravity_only:
+        print(json.dumps(run_gravity(args.n, args.steps, args.warmup, args.fp64_only)), flush=True)
         return
     p = ics.blast_params(args.n, solver=2, cfl=args.cfl)
     d = p.derive()
@@ -430,6 +476,8 @@ def main():
                                      log_path=os.path.join(ROOT, "gpurun_out", "batch_autotune.log"), accuracy=acc)
     if not args.no_conduction:
         forms["conduction"] = run_conduction(args.n, args.steps, args.warmup, args.fp64_only)
+    if not args.no_gravity:
+        forms["gravity"] = run_gravity(args.n, args.steps, args.warmup, args.fp64_only)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
     if os.path.exists(tpath):

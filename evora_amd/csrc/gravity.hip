@@ -194,9 +194,11 @@ __global__ void __launch_bounds__(GT) k_gravint(KParams P, GParams Gp, GGeo G, S
       f[1] = G.ny == 1 ? 0.0 : -(F[me + GRX] - F[me - GRX]) * Gp.dy2inv;
       f[2] = G.nz == 1 ? 0.0 : -(pn - pm) * Gp.dz2inv;
       const long long c = col + plane * k;
+      // q(6) enters only the dual-energy branch of the primitive below
       double qc[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++) qc[m] = double(q[m * G.n + c]);
+      for (int m = 0; m < 5; m++) qc[m] = double(q[m * G.n + c]);
+      qc[5] = P.dualenergy ? double(q[5 * G.n + c]) : 0.0;
       qc[4] = qc[4] + dt * (qc[1] * f[0] + qc[2] * f[1] + qc[3] * f[2]);
 #pragma unroll
       for (int d = 0; d < 3; d++) qc[1 + d] = qc[1 + d] + dt * qc[0] * f[d];
