@@ -175,3 +175,20 @@ def test_gravity_refusals():
     with hydro.Hydro(ics.blast_params(8).derive()) as h:
         with pytest.raises(hydro.EvoraError):
             h.gravpotential()
+
+
+def test_gravity_and_conduction_in_staged_mode():
+    """The per-routine kernels (EVH_MODE_STAGED) followed by the gravity and
+    conduction passes in the reference's arithmetic: 3 steps against the
+    oracle."""
+    p = ics.conduction_params(12, boundtype=1, gravity=True)
+    d = dataclasses.replace(p.derive(), grav=5e4)
+    q0 = ics.conduction_blob(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 3)
+    with hydro.Hydro(d, mode=hydro.MODE_STAGED) as h:
+        h.upload_q(q0)
+        h.run(3)
+        q = h.download_q()
+    for c in range(6):
+        assert rel_l1(q[..., c], qo[..., c]) <= 1e-12, c
