@@ -24,8 +24,9 @@ the force kernel timed, on a 512^3 Jeans-unstable box), and the reference's
 CPU hot path timed on the host's cores on a bounded sample.
 Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
 HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50 steps, the
-cosmological-like field 256^3 for 100 steps, and the batched ensemble) is
-also advanced in FP64 for the same number of steps, and the
+cosmological-like field 256^3 for 100 steps, the Jeans box with gravity and
+the hot blob with conduction at 256^3 for 50 steps, and the batched
+ensemble) is also advanced in FP64 for the same number of steps, and the
 relative L1 errors of rho, p and |v| and the mass and energy drift of both
 forms are reported under forms.*.accuracy_vs_fp64 (outside every timed
 region).
@@ -469,6 +470,12 @@ def main():
             accuracy["kh256"] = accuracy_case("kh", kp.derive(), ics.kelvin_helmholtz(kp), 50, control=True)
             cp = ics.cosmo_params(256)
             accuracy["cosmo256"] = accuracy_case("cosmo", cp.derive(), ics.cosmo_field(cp), 100, control=True)
+            jp = ics.jeans_params(256)
+            accuracy["jeans256_gravity"] = accuracy_case("jeans gravity", jp.derive(), ics.jeans(jp), 50,
+                                                         control=True)
+            bp = ics.conduction_params(256, boundtype=2)
+            accuracy["blob256_conduction"] = accuracy_case("blob conduction", bp.derive(), ics.conduction_blob(bp),
+                                                           50, control=True)
             forms["fp32_mixed"]["accuracy_vs_fp64"] = accuracy
     if not args.fp64_only and not args.no_batch:
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
