@@ -444,7 +444,7 @@ int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
   for (int i = 0; i < nbox; i++) {
     const evh_config &c = cfgs[i];
     if (!fused_supported(c)) return EVH_ERR_CONFIG;  // 3-D boxes only
-    if (c.conduction) return EVH_ERR_CONFIG;         // hydro step only
+    if (c.conduction || c.gravity) return EVH_ERR_CONFIG;  // hydro step only
     if (c.solver != 1 && c.solver != 2) return EVH_ERR_SOLVER;
     if (c.limiter < 1 || c.limiter > 3) return EVH_ERR_LIMITER;
     if (c.solver != c0.solver || c.limiter != c0.limiter || c.precision != c0.precision ||

@@ -164,7 +164,8 @@ long long evh_device_bytes(evh_ctx *h);
  * (box, tile) work list; with use_graph the pair is replayed from a hipGraph
  * and the host does not synchronise between steps.  All boxes share
  * solver, limiter, precision, secondorder and dualenergy; grids, spacing,
- * boundtype and cfl may differ. */
+ * boundtype and cfl may differ.  The batched form runs the hydro step only:
+ * boxes with conduction or gravity on are rejected (EVH_ERR_CONFIG). */
 typedef struct evh_batch evh_batch;
 int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out);
 int evh_batch_destroy(evh_batch *b);

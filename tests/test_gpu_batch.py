@@ -93,3 +93,11 @@ def test_batch_autotune_restores_state():
         plain = [b.download_q(i) for i in range(len(ps))]
     for a, c in zip(tuned, plain):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("kw", [dict(gravity=True), dict(conduction=True, simplekappa=True)])
+def test_batch_rejects_physics_beyond_the_hydro_step(kw):
+    params = [ics.blast_params(16), ics.blast_params(16, **kw)]
+    with pytest.raises(hydro.EvoraError) as ei:
+        Batch(params)
+    assert ei.value.code == 8
