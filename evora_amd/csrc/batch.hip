@@ -41,8 +41,6 @@ struct BoxDev {
   int pad;
 };
 
-constexpr int kXcds = 8;
-
 struct Queues {
   unsigned next[kXcds];  // items taken from each XCD's queue
   int start[kXcds + 1];  // queue j covers items [start[j], start[j+1])

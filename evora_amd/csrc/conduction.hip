@@ -175,14 +175,6 @@ struct CondMath<false> {
 
 using M = CondMath<kExact>;
 
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
-  const int j = b % kXcds, i = b / kXcds;
-  const int base = ntiles / kXcds, rem = ntiles % kXcds;
-  const int count = base + (j < rem ? 1 : 0);
-  if (i >= count) return -1;
-  return j * base + (j < rem ? j : rem) + i;
-}
 
 template <typename S>
 __global__ void __launch_bounds__(CT) k_conduct(KParams P, CGeo G, S *__restrict__ q, const S *__restrict__ tb,
@@ -329,7 +321,7 @@ template <typename S>
 void launch_conduction_s(evh_ctx *h, double dt) {
   hipLaunchKernelGGL(k_reset_tv, dim3(1), dim3(64), 0, h->stream, h->st);
   const CGeo g = make_cgeo(h);
-  const int blocks = kXcds * ((g.ntiles + kXcds - 1) / kXcds);
+  const int blocks = xcd_blocks(g.ntiles);
   hipLaunchKernelGGL(k_conduct<S>, dim3(blocks), dim3(CT), 0, h->stream, h->kp, g, (S *)h->qbuf[h->cur],
                      (const S *)h->tbuf, dt, h->st);
 }

@@ -7,6 +7,7 @@
 
 #include "../../include/evora_hip.h"
 #include "physics.h"
+#include "tiling.h"
 #include "transfer.h"
 
 namespace evh {

@@ -24,19 +24,6 @@ namespace {
 
 using core::Geo;
 
-// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Block b runs on XCD b % 8; giving XCD j the contiguous run of tiles
-// [j*T/8, (j+1)*T/8) (x fastest, then y, then z chunk) keeps the halo
-// columns a block shares with its x and y neighbours in one L2.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
-  const int j = b % kXcds, i = b / kXcds;
-  const int base = ntiles / kXcds, rem = ntiles % kXcds;
-  const int count = base + (j < rem ? 1 : 0);
-  if (i >= count) return -1;
-  return j * base + (j < rem ? j : rem) + i;
-}
-
 // WPE: waves per SIMD the register allocation must leave room for
 // (512 / WPE VGPRs per lane); the tuning knob next to the tile height TY.
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
@@ -84,7 +71,7 @@ Geo make_geo(const evh_ctx *h, int ty) {
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 void launch_ty(evh_ctx *h, double dt) {
   const Geo g = make_geo(h, TY);
-  const int blocks = kXcds * ((g.ntiles + kXcds - 1) / kXcds);
+  const int blocks = xcd_blocks(g.ntiles);
   hipLaunchKernelGGL((k_fused<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
                      h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
                      (S *)h->tbuf);

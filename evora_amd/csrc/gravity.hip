@@ -40,7 +40,6 @@ constexpr int GX = 64, GY = 4, GT = GX * GY;  // force tile and threads
 constexpr int GRX = GX + 2, GRY = GY + 2;
 constexpr int GRING = 2 * GX + 2 * GY;
 constexpr int kZChunk = 128;
-constexpr int kXcds = 8;
 
 struct GParams {
   double grav, fb, h2, n3;
@@ -53,14 +52,6 @@ struct GGeo {
   long long n;
   int zchunk, tiles_x, tiles_y, ntiles;
 };
-
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
-  const int j = b % kXcds, i = b / kXcds;
-  const int base = ntiles / kXcds, rem = ntiles % kXcds;
-  const int count = base + (j < rem ? 1 : 0);
-  if (i >= count) return -1;
-  return j * base + (j < rem ? j : rem) + i;
-}
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -258,7 +249,7 @@ hipError_t fft_status(hipfftResult r) { return r == HIPFFT_SUCCESS ? hipSuccess 
 template <typename S, bool EXACT>
 void launch_gravint_t(evh_ctx *h, double dt) {
   const GGeo g = make_ggeo(h);
-  const int blocks = kXcds * ((g.ntiles + kXcds - 1) / kXcds);
+  const int blocks = xcd_blocks(g.ntiles);
   hipLaunchKernelGGL((k_gravint<S, EXACT>), dim3(blocks), dim3(GT), 0, h->stream, h->kp, make_gparams(h), g,
                      (S *)h->qbuf[h->cur], h->phi, dt, h->st, (S *)h->tbuf);
 }
