@@ -41,6 +41,24 @@ def test_potential_matches_oracle(shape):
     assert rel_linf(phi, ref) <= 1e-12
 
 
+def test_potential_unequal_spacing():
+    """dx != dy != dz: multigreen's h2 is dx**2 / 4 whatever the other
+    spacings (gravitation.f90:80, parameter.f90:254), the force divides by
+    each direction's own 2 dx."""
+    p = ics.jeans_params(12, ny=10, nz=9, xmax=2.0, ymax=1.5, zmax=0.8)
+    d = p.derive()
+    q = ics.jeans(p)
+    assert rel_linf(_potential(d, q), O.Oracle(d).gravpotential(q)) <= 1e-12
+    qo = q.copy()
+    O.Oracle(d).run(qo, 2)
+    with hydro.Hydro(d) as h:
+        h.upload_q(q)
+        h.run(2)
+        qg = h.download_q()
+    for c in range(6):
+        assert rel_linf(qg[..., c], qo[..., c]) <= 1e-12, c
+
+
 def test_potential_supercomoving_source():
     p = HydroParams(nx=16, ny=12, nz=10, gravity=True, supercomoving=True, fb=0.17, atomic=True)
     d = p.derive()
