@@ -18,6 +18,13 @@
 #error "the exact form is FP64"
 #endif
 constexpr bool kExact = EVH_FUSED_EXACT != 0;
+// the exact build's kernel has a name of its own, so that profiles keep the
+// two FP64 forms apart
+#if EVH_FUSED_EXACT
+#define K_FUSED k_fused_exact
+#else
+#define K_FUSED k_fused
+#endif
 
 namespace evh {
 namespace {
@@ -28,7 +35,7 @@ using core::Geo;
 // (512 / WPE VGPRs per lane); the tuning knob next to the tile height TY.
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-    k_fused(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
+    K_FUSED(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
             S *__restrict__ tout) {
   __shared__ core::Smem<T, TY> sm;
   const int lin = xcd_tile(blockIdx.x, G.ntiles);
@@ -72,7 +79,7 @@ template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 void launch_ty(evh_ctx *h, double dt) {
   const Geo g = make_geo(h, TY);
   const int blocks = xcd_blocks(g.ntiles);
-  hipLaunchKernelGGL((k_fused<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
+  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
                      h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
                      (S *)h->tbuf);
 }
