@@ -175,7 +175,6 @@ struct CondMath<false> {
 
 using M = CondMath<kExact>;
 
-
 template <typename S>
 __global__ void __launch_bounds__(CT) k_conduct(KParams P, CGeo G, S *__restrict__ q, const S *__restrict__ tb,
                                                 double dt, Status *st) {
