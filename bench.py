@@ -421,6 +421,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--batch-boxes", type=int, default=48)
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--autotune-log", default=None,
+                    help="file to write the batched form's autotuner table to (the chosen shape is in the JSON)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -478,9 +480,10 @@ def main():
                                                            50, control=True)
             forms["fp32_mixed"]["accuracy_vs_fp64"] = accuracy
     if not args.fp64_only and not args.no_batch:
-        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-        forms["batched"] = run_batch(args.batch_boxes, args.batch_steps, 2,
-                                     log_path=os.path.join(ROOT, "gpurun_out", "batch_autotune.log"), accuracy=acc)
+        if args.autotune_log:
+            os.makedirs(os.path.dirname(os.path.abspath(args.autotune_log)), exist_ok=True)
+        forms["batched"] = run_batch(args.batch_boxes, args.batch_steps, 2, log_path=args.autotune_log,
+                                     accuracy=acc)
     if not args.no_conduction:
         forms["conduction"] = run_conduction(args.n, args.steps, args.warmup, args.fp64_only)
     if not args.no_gravity:

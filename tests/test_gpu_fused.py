@@ -1,6 +1,6 @@
 """GPU parity of the fused one-pass FP64 step against the CPU oracle.
 
-Bar (north star): relative L-infinity <= 1e-12 per component after one
+Bar: relative L-infinity <= 1e-12 per component after one
 step, relative L1 <= 1e-10 after a multi-step run.  rel L-inf of a component
 is max|gpu - oracle| / max|oracle|.
 """
