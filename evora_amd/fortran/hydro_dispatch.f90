@@ -11,7 +11,7 @@
 !   EVORA_BACKEND=cpu (default)  calls the reference routines unchanged;
 !   EVORA_BACKEND=hip            runs the step on the GPU through libevora_hip.
 !
-! It also defines evolution (snapshot.f90:159-193), the driver's periodic
+! It also defines evolution (snapshot.f90:159-193), main.f90's periodic
 ! report of total mass, energy and entropy: once the state lives on the
 ! device the three sums are taken there (evh_sums, FP64 compensated), so an
 ! evolution step moves 24 bytes instead of the whole of q.

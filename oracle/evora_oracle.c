@@ -1,7 +1,7 @@
 /*
  * evora_oracle.c -- CPU restatement of evora's hydro step (TEST INFRASTRUCTURE).
  *
- * The checker for the HIP library, never the product.  Every function keeps
+ * The reference the HIP library is tested against, never the product.  Every function keeps
  * the reference's operand order so that, compiled without FP contraction,
  * it reproduces the compiled Fortran bit for bit (pinned against oracle/_ref
  * by tests/test_oracle_ref.py).  OpenMP only splits independent cells or

@@ -1,7 +1,7 @@
 /*
  * evora_oracle.h -- CPU restatement of evora's hydro step (TEST INFRASTRUCTURE).
  *
- * This is the checker, never the product: only tests/, __graft_entry__.smoke()
+ * This is test infrastructure, never the product: only tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg may load it.  It restates, in plain C with
  * the reference's evaluation order, the routines of
  *   source/comp.f90:26-85        q2w, pressure

@@ -1,4 +1,4 @@
-"""Python access to the CPU checker (TEST INFRASTRUCTURE).
+"""Python access to the CPU oracle (TEST INFRASTRUCTURE).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
 this module.  It wraps
