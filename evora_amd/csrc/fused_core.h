@@ -291,11 +291,15 @@ template <typename S>
 __device__ __forceinline__ void fetch_raw(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
                                           int gz, S raw[6]) {
   const bool per = P.boundtype == 2;
+  // clamped into the ghost layers: partial tiles in x and y, and in z the
+  // prefetch of plane nz + 2 after the last plane (never used; it would fall
+  // outside the grid for nz = 2)
   const int cx = gx < -2 ? -2 : (gx > G.nx + 1 ? G.nx + 1 : gx);
   const int cy = gy < -2 ? -2 : (gy > G.ny + 1 ? G.ny + 1 : gy);
+  const int cz = gz < -2 ? -2 : (gz > G.nz + 1 ? G.nz + 1 : gz);
   const int sx = ghost_src(cx, G.nx, per);
   const int sy = ghost_src(cy, G.ny, per);
-  const int sz = ghost_src(gz, G.nz, per);
+  const int sz = ghost_src(cz, G.nz, per);
   const long long c = (long long)sx + (long long)G.nx * ((long long)sy + (long long)G.ny * sz);
 #pragma unroll
   for (int m = 0; m < 6; m++) raw[m] = q[m * G.n + c];

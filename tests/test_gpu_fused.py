@@ -53,7 +53,7 @@ def test_fused_one_step(solver, limiter, bt):
 
 @pytest.mark.parametrize("shape", [(30, 28, 20), (16, 45, 9), (64, 16, 70)])
 def test_fused_tile_edges(shape):
-    """Grids that are not multiples of the 14x14 tile or the z chunk."""
+    """Grids that are not multiples of the 16x12 tile or the z chunk."""
     nx, ny, nz = shape
     p = HydroParams(nx=nx, ny=ny, nz=nz, solver=2, limiter=1, boundtype=1, cfl=0.4, atomic=True)
     q0 = ics.random_smooth(p, seed=nx + ny + nz)
@@ -130,3 +130,33 @@ def test_fused_matches_staged():
             out.append(h.download_q())
     for c in range(6):
         assert rel_linf(out[1][..., c], out[0][..., c]) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 2, 2), (2, 5, 3), (5, 3, 2), (17, 2, 9), (2, 13, 2)])
+@pytest.mark.parametrize("bt", [1, 2, 3])
+def test_fused_smallest_grids(shape, bt):
+    """Grids down to the fused path's minimum of 2 cells per direction, where
+    the two ghost layers span the whole grid and the march's prefetch reaches
+    past the last plane: 3 steps of the fast form (relative L-infinity
+    <= 1e-12), the exact form (density, momentum and energy bit for bit) and
+    FP32-mixed (relative L1 of rho and p <= 1e-5) against the oracle."""
+    nx, ny, nz = shape
+    p = HydroParams(nx=nx, ny=ny, nz=nz, solver=2, limiter=2, boundtype=bt, cfl=0.4, atomic=True)
+    d = p.derive()
+    q0 = ics.random_smooth(p, seed=nx * 100 + ny * 10 + nz + bt, jumps=False)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 3)
+    for mode, prec in ((hydro.MODE_FUSED, hydro.FP64), (hydro.MODE_FUSED_EXACT, hydro.FP64),
+                       (hydro.MODE_FUSED, hydro.FP32_MIXED)):
+        with hydro.Hydro(d, precision=prec, mode=mode) as h:
+            h.upload_q(q0)
+            h.run(3)
+            q = h.download_q()
+        if prec == hydro.FP32_MIXED:
+            for c in (0, 4):
+                assert rel_l1(q[..., c], qo[..., c]) <= 1e-5, (mode, prec, c)
+        elif mode == hydro.MODE_FUSED_EXACT:
+            assert np.array_equal(q[..., :5], qo[..., :5])
+        else:
+            for c in range(6):
+                assert rel_linf(q[..., c], qo[..., c]) <= 1e-12, (mode, prec, c)
