@@ -164,3 +164,20 @@ def test_conduction_fullsize_conservation_and_forms():
         assert s1[1] == pytest.approx(s0[1], rel=1e-13)
     for c in range(6):
         assert rel_linf(out[hydro.MODE_FUSED][..., c], out[hydro.MODE_FUSED_EXACT][..., c]) <= 1e-12, c
+
+
+@pytest.mark.parametrize("shape", [(2, 3, 2), (3, 2, 2)])
+def test_conduction_smallest_grids(shape):
+    """2 and 3 cells per direction: the ring and the z neighbours of the
+    conduction pass come from the ghost rules across the whole grid."""
+    p = ics.conduction_params(2, nx=shape[0], ny=shape[1], nz=shape[2], boundtype=1)
+    d = p.derive()
+    q0 = ics.conduction_blob(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 2)
+    with hydro.Hydro(d) as h:
+        h.upload_q(q0)
+        h.run(2)
+        q = h.download_q()
+    for c in range(6):
+        assert rel_linf(q[..., c], qo[..., c]) <= 1e-12, c

@@ -210,3 +210,20 @@ def test_gravity_and_conduction_in_staged_mode():
         q = h.download_q()
     for c in range(6):
         assert rel_l1(q[..., c], qo[..., c]) <= 1e-12, c
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 2, 2), (2, 3, 2)])
+def test_gravity_smallest_grids(shape):
+    """Transforms of length 2 and 3 and a force stencil that wraps onto the
+    cell's own neighbours: 2 steps against the oracle."""
+    p = ics.jeans_params(2, nx=shape[0], ny=shape[1], nz=shape[2])
+    d = p.derive()
+    q0 = ics.jeans(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 2)
+    with hydro.Hydro(d) as h:
+        h.upload_q(q0)
+        h.run(2)
+        q = h.download_q()
+    for c in range(6):
+        assert rel_linf(q[..., c], qo[..., c]) <= 1e-12, c

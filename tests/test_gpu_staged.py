@@ -102,3 +102,23 @@ def test_staged_sod_run_matches_reference_fixture(run_fixtures, golden_index):
     assert n == len(t_dt)
     assert t == t_dt[-1][0] and dt == t_dt[-1][1]
     assert_q_parity(q, run_fixtures["sod1000/q"], tol6=1e-13)
+
+
+@pytest.mark.parametrize("shape", [(2, 1, 1), (3, 1, 1), (2, 2, 1), (3, 2, 1), (2, 5, 1), (2, 2, 2), (3, 2, 2)])
+@pytest.mark.parametrize("bt", [1, 2, 3])
+def test_staged_smallest_grids(shape, bt):
+    """Grids of 2 and 3 cells per direction, where the two ghost layers of
+    `bound` span the whole grid: 3 staged steps against the oracle, bit for
+    bit except the entropy."""
+    nx, ny, nz = shape
+    p = HydroParams(nx=nx, ny=ny, nz=nz, solver=2, limiter=1, boundtype=bt, cfl=0.4, atomic=True)
+    d = p.derive()
+    q0 = ics.random_smooth(p, seed=nx + ny + nz + bt, jumps=False)
+    qo = q0.copy()
+    to, dto, _ = O.Oracle(d).run(qo, 3)
+    with hydro.Hydro(d, mode=hydro.MODE_STAGED) as h:
+        h.upload_q(q0)
+        t, dt, n = h.run(3)
+        q = h.download_q()
+    assert n == 3 and t == to and dt == dto
+    assert_q_parity(q, qo)
