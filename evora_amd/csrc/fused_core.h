@@ -9,22 +9,22 @@
 // halo ring and loads the second ring and the four corners, so every
 // interior lane does useful work in every stage.  Per plane k:
 //   S1  primitive (comp.f90:26-85) of plane k+1 for every owned column, with
-//       the ghost cells of `bound` (boundary.f90) folded into the load index;
-//       plane k goes to LDS with both halo rings.
+//       the ghost cells of `bound` (boundary.f90) folded into the load index
+//       (a column's x-y offset is formed once per march); plane k goes to
+//       LDS with both halo rings.
 //   S2  limited slopes in x, y, z and the predictor evo (muscl.f90:45-169)
 //       of plane k; the x/y face states go to LDS per face; the interior
 //       lanes solve their z face k-1/2 (riemann.f90) and finish plane k-1
 //       as (q + dtx*(Fx) + dty*(Fy)) + dtz*(Fz), the reference's x, y, z
-//       order (muscl.f90:191-272, integration :287); primitive + the CFL
-//       limit of the new state feed a min reduction for the next hydrostep
-//       (timestep.f90:26-57).
+//       order (muscl.f90:191-272, integration :287), and store it.
 //   S3  x and y faces of plane k: interior lane (i,j) solves its left and
-//       bottom faces, the ring wave the TY right-edge and 16 top-edge faces.
-//       Each flux overwrites the two states it came from.
-//   S4  q(k) + dtx*(Fx) + dty*(Fy) of plane k, kept for S2 of plane k+1.
-// What a column carries from plane to plane besides its w window (the
-// partial update, the z flux and z state of the face below) lives in
-// lane-private LDS, so no register holds it through the x/y solves.
+//       bottom faces, the ring wave the TY right-edge and 16 top-edge faces
+//       in one solve.  Each flux overwrites the two states it came from.
+//   S4  q(k) + dtx*(Fx) + dty*(Fy) of plane k, kept for S2 of plane k+1;
+//       then the primitive and the CFL limit of the stored plane k-1, which
+//       feed a min reduction for the next hydrostep (timestep.f90:26-57).
+//       Here they fill the time the interior waves would otherwise wait at
+//       the next barrier for the ring wave's two primitives of S1.
 // Template parameters: T compute type, S storage type of q (double/double
 // for the FP64 form, float/float for the FP32-mixed form), SOLVER, LIMITER,
 // TY the tile height (4, 8 or 12).
@@ -286,23 +286,34 @@ struct Phys<true> {
   }
 };
 
-// raw q at (gx, gy, gz), ghost-mapped (the index rules of `bound`)
+// Ghost-mapped offsets (the index rules of `bound`) of column (gx, gy) and
+// of plane gz, clamped into the ghost layers: partial tiles in x and y, and
+// in z the prefetch of plane nz + 2 after the last plane (never used; it
+// would fall outside the grid for nz = 2).  A column's offset is fixed for
+// the whole march, so only the plane's is formed per plane.
+__device__ __forceinline__ long long col_offset(const KParams &P, const Geo &G, int gx, int gy) {
+  const bool per = P.boundtype == 2;
+  const int cx = gx < -2 ? -2 : (gx > G.nx + 1 ? G.nx + 1 : gx);
+  const int cy = gy < -2 ? -2 : (gy > G.ny + 1 ? G.ny + 1 : gy);
+  return (long long)ghost_src(cx, G.nx, per) + (long long)G.nx * ghost_src(cy, G.ny, per);
+}
+__device__ __forceinline__ long long plane_offset(const KParams &P, const Geo &G, int gz) {
+  const int cz = gz < -2 ? -2 : (gz > G.nz + 1 ? G.nz + 1 : gz);
+  return (long long)G.nx * G.ny * ghost_src(cz, G.nz, P.boundtype == 2);
+}
+
+// raw q of the cell at offset c
+template <typename S>
+__device__ __forceinline__ void fetch_at(const Geo &G, const S *__restrict__ q, long long c, S raw[6]) {
+#pragma unroll
+  for (int m = 0; m < 6; m++) raw[m] = q[m * G.n + c];
+}
+
+// raw q at (gx, gy, gz), ghost-mapped
 template <typename S>
 __device__ __forceinline__ void fetch_raw(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
                                           int gz, S raw[6]) {
-  const bool per = P.boundtype == 2;
-  // clamped into the ghost layers: partial tiles in x and y, and in z the
-  // prefetch of plane nz + 2 after the last plane (never used; it would fall
-  // outside the grid for nz = 2)
-  const int cx = gx < -2 ? -2 : (gx > G.nx + 1 ? G.nx + 1 : gx);
-  const int cy = gy < -2 ? -2 : (gy > G.ny + 1 ? G.ny + 1 : gy);
-  const int cz = gz < -2 ? -2 : (gz > G.nz + 1 ? G.nz + 1 : gz);
-  const int sx = ghost_src(cx, G.nx, per);
-  const int sy = ghost_src(cy, G.ny, per);
-  const int sz = ghost_src(cz, G.nz, per);
-  const long long c = (long long)sx + (long long)G.nx * ((long long)sy + (long long)G.ny * sz);
-#pragma unroll
-  for (int m = 0; m < 6; m++) raw[m] = q[m * G.n + c];
+  fetch_at(G, q, col_offset(P, G, gx, gy) + plane_offset(P, G, gz), raw);
 }
 
 // primitive of a fetched cell at column gx (for the reflecting wall)
@@ -410,15 +421,23 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   // first Riemann error and first negative pressure of the new state seen by
   // this lane (code, plane); reported once after the march
   int err = 0, err_k = 0, nerr = 0, nerr_k = 0;
+  // q^{n+1} of plane k-1, stored in S2; its primitive, CFL limit and
+  // temperature are formed after the x/y solves (S4), where the interior
+  // waves would otherwise wait for the ring wave
+  T o[6];
+  bool pend = false;
+  long long oc = 0;
 
   // global loads run one plane ahead: pre holds the raw q of this column's
   // next plane and pre2 the ring wave's second-ring cell, so that no wave
   // stalls on memory right after issuing a load
   S pre[6], pre2[6];
+  const long long own_xy = col_offset(P, G, gx, gy);
+  const long long r2_xy = col_offset(P, G, x0 + r2x, y0 + r2y);
   load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
   load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
-  fetch_raw(P, G, qin, gx, gy, z0, pre);
-  fetch_raw(P, G, qin, x0 + r2x, y0 + r2y, z0 - 1, pre2);
+  fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
+  fetch_at(G, qin, r2_xy + plane_offset(P, G, z0 - 1), pre2);
 #pragma unroll
   for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
 
@@ -427,13 +446,13 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
     if (owns_col) {
       prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
-      fetch_raw(P, G, qin, gx, gy, k + 2, pre);
+      fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
     }
     if (ring && r2x > -3) {
       T rw[5], rr;
       prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw, rr);
       store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw);
-      fetch_raw(P, G, qin, x0 + r2x, y0 + r2y, k + 1, pre2);
+      fetch_at(G, qin, r2_xy + plane_offset(P, G, k + 1), pre2);
     }
     if (owns_col) store_w<T, EA>(W, me, wc);
     __syncthreads();
@@ -507,19 +526,12 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
         const int e = Phys<EXACT>::template riemann<SOLVER, 3>(P, zrp, zl, fz);
         if (e && inside && !err) { err = e; err_k = k; }
         if (k - 1 >= z0 && inside) {
-          T o[6];
 #pragma unroll
           for (int m = 0; m < 6; m++) o[m] = CR[m][tid] + dtz * (CR[6 + m][tid] - fz[m]);
-          const long long c = (long long)gx + (long long)G.nx * ((long long)gy + (long long)G.ny * (k - 1));
+          oc = (long long)gx + (long long)G.nx * ((long long)gy + (long long)G.ny * (k - 1));
 #pragma unroll
-          for (int m = 0; m < 6; m++) qout[m * G.n + c] = S(o[m]);
-          T on[6], wn[5], rn;
-#pragma unroll
-          for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
-          const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
-          if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
-          vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
-          if (tout) tout[c] = S(Phys<EXACT>::temp(P, wn, rn));
+          for (int m = 0; m < 6; m++) qout[m * G.n + oc] = S(o[m]);
+          pend = true;
         }
 #pragma unroll
         for (int m = 0; m < 6; m++) CR[6 + m][tid] = fz[m];
@@ -557,27 +569,30 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
           YB[m][fyi] = fy[m];
         }
       } else {
-        // ring wave: the TY right-edge x faces and the 16 top-edge y faces
-        int fxi = -1, fyi = -1;
-        if (lane < TY) fxi = TX + (TX + 1) * lane;
-        else if (lane < TY + TX) fyi = (lane - TY) + TX * TY;
-        if (fxi >= 0) {
+        // ring wave: the TY right-edge x faces (lanes 0..TY-1) and the 16
+        // top-edge y faces (lanes TY..TY+15) in one solve, so that the wave
+        // does not run two: a y face is the x face of the states with u and
+        // v exchanged (the solvers are symmetric under that exchange)
+        if (lane < TY + TX) {
+          const bool isy = lane >= TY;
+          T *fb = isy ? &YB[0][(lane - TY) + TX * TY] : &XB[0][TX + (TX + 1) * lane];
+          const int st_ = isy ? TS::NFY : TS::NFX;
           T wl[5], wr[5], f[6];
 #pragma unroll
-          for (int m = 0; m < 5; m++) { wl[m] = XB[m][fxi]; wr[m] = XB[5 + m][fxi]; }
+          for (int m = 0; m < 5; m++) { wl[m] = fb[m * st_]; wr[m] = fb[(5 + m) * st_]; }
+          const T l1 = wl[1], r1 = wr[1];
+          wl[1] = isy ? wl[2] : wl[1];
+          wl[2] = isy ? l1 : wl[2];
+          wr[1] = isy ? wr[2] : wr[1];
+          wr[2] = isy ? r1 : wr[2];
           const int e = Phys<EXACT>::template riemann<SOLVER, 1>(P, wl, wr, f);
-          if (e && !err && x0 + TX <= G.nx && y0 + lane < G.ny) { err = e; err_k = k; }
+          const T f1 = f[1];
+          f[1] = isy ? f[2] : f[1];
+          f[2] = isy ? f1 : f[2];
+          const bool counts = isy ? (x0 + lane - TY < G.nx && y0 + TY <= G.ny) : (x0 + TX <= G.nx && y0 + lane < G.ny);
+          if (e && !err && counts) { err = e; err_k = k; }
 #pragma unroll
-          for (int m = 0; m < 6; m++) XB[m][fxi] = f[m];
-        }
-        if (fyi >= 0) {
-          T wl[5], wr[5], f[6];
-#pragma unroll
-          for (int m = 0; m < 5; m++) { wl[m] = YB[m][fyi]; wr[m] = YB[5 + m][fyi]; }
-          const int e = Phys<EXACT>::template riemann<SOLVER, 2>(P, wl, wr, f);
-          if (e && !err && x0 + lane - TY < G.nx && y0 + TY <= G.ny) { err = e; err_k = k; }
-#pragma unroll
-          for (int m = 0; m < 6; m++) YB[m][fyi] = f[m];
+          for (int m = 0; m < 6; m++) fb[m * st_] = f[m];
         }
       }
     }
@@ -591,6 +606,18 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
 #pragma unroll
       for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+    }
+    // the end of plane k-1: primitive of the new state for the CFL minimum
+    // of the next step, its first negative pressure, its temperature
+    if (pend) {
+      T on[6], wn[5], rn;
+#pragma unroll
+      for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
+      const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
+      if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
+      vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
+      if (tout) tout[oc] = S(Phys<EXACT>::temp(P, wn, rn));
+      pend = false;
     }
     // rotate the column window
 #pragma unroll

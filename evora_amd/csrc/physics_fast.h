@@ -229,7 +229,10 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   f[4] = ud * (energy + w[4]);  // ud (0.5 rho u2 + gg1 p) with gg1 = 1 + 1/g1
   f[5] = ud * sq;
   const T s = right ? sr : sl;
-  const T isd = fm::rcp(s - sm);
+  const bool star = br >= 2;
+  // outside the star regions the correction is multiplied by 0; its
+  // divisor is kept finite there
+  const T isd = fm::rcp(star ? s - sm : T(1));
   const T r = (s - ud) * isd;
   const T qx0 = w[0] * r;
   T dq[6];
@@ -238,9 +241,9 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   for (int i = 1; i <= 3; i++) dq[i] = ((i == DIR) ? sm : w[i]) * qx0 - w[0] * w[i];
   dq[4] = r * energy + (sm - ud) * (qx0 * sm + w[4] * isd) - energy;
   dq[5] = r * sq - sq;
-  const bool star = br >= 2;
+  const T ss = star ? s : T(0);
 #pragma unroll
-  for (int i = 0; i < 6; i++) f[i] = star ? f[i] + s * dq[i] : f[i];
+  for (int i = 0; i < 6; i++) f[i] = f[i] + ss * dq[i];
   return code;
 }
 
