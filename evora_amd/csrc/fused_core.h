@@ -11,7 +11,8 @@
 //   S1  primitive (comp.f90:26-85) of plane k+1 for every owned column, with
 //       the ghost cells of `bound` (boundary.f90) folded into the load index
 //       (a column's x-y offset is formed once per march); plane k goes to
-//       LDS with both halo rings.
+//       LDS with both halo rings.  The ring wave's second-ring cells of
+//       plane k were formed in its idle time of S2 of plane k-1.
 //   S2  limited slopes in x, y, z and the predictor evo (muscl.f90:45-169)
 //       of plane k; the x/y face states go to LDS per face; the interior
 //       lanes solve their z face k-1/2 (riemann.f90) and finish plane k-1
@@ -23,8 +24,7 @@
 //   S4  q(k) + dtx*(Fx) + dty*(Fy) of plane k, kept for S2 of plane k+1;
 //       then the primitive and the CFL limit of the stored plane k-1, which
 //       feed a min reduction for the next hydrostep (timestep.f90:26-57).
-//       Here they fill the time the interior waves would otherwise wait at
-//       the next barrier for the ring wave's two primitives of S1.
+//       They sit where the interior waves have the least to do.
 // Template parameters: T compute type, S storage type of q (double/double
 // for the FP64 form, float/float for the FP32-mixed form), SOLVER, LIMITER,
 // TY the tile height (4, 8 or 12).
@@ -437,23 +437,27 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
   load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
   fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
-  fetch_at(G, qin, r2_xy + plane_offset(P, G, z0 - 1), pre2);
+  // the ring wave's second-ring cell: its w of the current plane (rw2) is
+  // formed one plane ahead, in the ring wave's idle time of S2
+  T rw2[5];
+  {
+    T rr2;
+    fetch_at(G, qin, r2_xy + plane_offset(P, G, z0 - 1), pre2);
+    prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
+    fetch_at(G, qin, r2_xy + plane_offset(P, G, z0), pre2);
+  }
 #pragma unroll
   for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
 
   for (int k = z0 - 1; k <= z1; k++) {
     const bool out_plane = k >= z0 && k < z1;
-    // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS
-    if (owns_col) {
-      prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
-      fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
-    }
-    if (ring && r2x > -3) {
-      T rw[5], rr;
-      prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw, rr);
-      store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw);
-      fetch_at(G, qin, r2_xy + plane_offset(P, G, k + 1), pre2);
-    }
+    // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS.
+    // Every lane forms and fetches (the ring wave's corner and idle lanes on
+    // clamped coordinates), so that no branch makes a load wait for the
+    // fetches issued after it.
+    prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
+    fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
+    if (ring && r2x > -3) store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw2);
     if (owns_col) store_w<T, EA>(W, me, wc);
     __syncthreads();
 
@@ -540,6 +544,11 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       for (int m = 0; m < 5; m++) CR[12 + m][tid] = zr[m];
     }
     }  // owns_col
+    if (ring) {  // second-ring w of plane k+1, stored in S1 of plane k+1
+      T rr2;
+      prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
+      fetch_at(G, qin, r2_xy + plane_offset(P, G, k + 2), pre2);
+    }
     __syncthreads();
 
     // ---- S3: x and y faces of plane k
