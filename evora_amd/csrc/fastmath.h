@@ -101,21 +101,31 @@ __device__ __forceinline__ double pow_pos(double x, double y) { return exp_fin(y
 // Powers x**(-g1) and x**g1 when g1 = 2/3 - delta with |delta| <= 1e-6 (the
 // single-precision 5/3 of an atomic gas gives delta = 3.97e-8), for
 // positive x = m 2^e with |e| < 200:
-//   y = x**(-1/3): single-precision estimate, then two Newton steps
-//       y += (y/3)(1 - x y^3), each squaring the relative error
-//       (~1e-6 -> ~2e-12 -> rounding);
+//   y = x**(-1/3): single-precision estimate (relative error below 4e-6:
+//       the single-precision sum e + log2 m carries |e| <= 200), then one
+//       Halley step: with t = 1 - x y^3, x**(-1/3) = y (1-t)**(-1/3)
+//       = y (1 + t/3 + 2t^2/9 + 14t^3/81 + ...), and the dropped terms are
+//       below 3e-16 for |t| <= 1.2e-5 (HALLEY = false: two Newton steps
+//       y += (y/3)(1 - x y^3) instead, each squaring the relative error;
+//       they hold fewer values at once);
 //   x**(-g1) = y^2 exp(c),  x**g1 = (x y) exp(-c),  c = delta ln x,
 //   with exp(c) = 1 + c + c^2/2 + c^3/6 (|c| < 1.4e-4, the dropped term is
 //   below 2e-17).  ln x enters only through c, so log2 of the mantissa in
 //   single precision (absolute error ~1e-7, times delta ln 2) is enough:
 //   relative error below 1e-14 (measured 8e-15 over 2^(+-200)).
 // third_root returns y and c; the caller checks the exponent range.
+template <bool HALLEY = true>
 __device__ __forceinline__ void third_root(double x, int e, double dln2, double &y, double &c) {
   const double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
   const float l2m = __builtin_amdgcn_logf(float(m));
   y = double(__builtin_amdgcn_exp2f((float(e) + l2m) * (-1.0f / 3.0f)));
-  y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x, y * y * y, 1.0), y);
-  y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x, y * y * y, 1.0), y);
+  if (HALLEY) {
+    const double t = __builtin_fma(-x, y * y * y, 1.0);
+    y = __builtin_fma(y * t, __builtin_fma(t, 2.0 / 9.0, 1.0 / 3.0), y);
+  } else {  // two Newton steps y += (y/3)(1 - x y^3): fewer live values
+    y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x, y * y * y, 1.0), y);
+    y = __builtin_fma(y * (1.0 / 3.0), __builtin_fma(-x, y * y * y, 1.0), y);
+  }
   c = dln2 * (double(e) + double(l2m));
 }
 

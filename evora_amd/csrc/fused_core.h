@@ -92,7 +92,9 @@ struct Prim<float> {
     if (P.dualenergy && kin * fm::rcp(q[4]) > float(P.de_thresh1)) {
       const double r64 = double(q[0]);
       const double k64 = 0.5 * (double(q[1]) * q[1] + double(q[2]) * q[2] + double(q[3]) * q[3]) / r64;
-      const double p64 = double(q[5]) * fast::pow_g1(P, r64);
+      // (the Newton form of the power: this rare branch must not raise the
+      // register count of the whole FP32 kernel)
+      const double p64 = double(q[5]) * fast::pow_g1<false>(P, r64);
       q[4] = float(k64 + p64 * P.g1inv);
       pr = float(p64);
     } else {
@@ -417,7 +419,11 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   // the update, kept in registers so that q is read from HBM once
   T wm[5], wc[5], wp[5], qn[6], qc[6], tq[6];
   T rm, rc, rp;
-  double vmin = 1e6;
+  // The CFL minimum min_c 1 / max_d (|u_d| + a) / dx_d.  The FP64 fast
+  // form reduces -max_d (..) per cell and inverts once per lane; a lane
+  // without cells gives 1e6 either way.
+  constexpr bool kSpeedKey = !EXACT && sizeof(T) == 8;
+  double vmin = kSpeedKey ? -1e-6 : 1e6;
   // first Riemann error and first negative pressure of the new state seen by
   // this lane (code, plane); reported once after the march
   int err = 0, err_k = 0, nerr = 0, nerr_k = 0;
@@ -624,7 +630,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
       const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
       if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
-      vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
+      if constexpr (kSpeedKey) vmin = dmin(vmin, -fast::cfl_speed(P, wn, rn));
+      else vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
       if (tout) tout[oc] = S(Phys<EXACT>::temp(P, wn, rn));
       pend = false;
     }
@@ -642,6 +649,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     if (err) report(st, err, tag | (unsigned long long)(ex + G.nx * (ey + (long long)G.ny * err_k)));
     if (nerr) report_next(st, nerr, tag | (unsigned long long)(ex + G.nx * (ey + (long long)G.ny * nerr_k)));
   }
+  if constexpr (kSpeedKey) vmin = fm::rcp(-vmin);
   for (int off = 32; off > 0; off >>= 1) vmin = dmin(vmin, __shfl_xor(vmin, off));
   if ((tid & 63) == 0) atomic_min_pos(vmin_dst, vmin);
 }

@@ -32,11 +32,12 @@ __device__ __forceinline__ double rpow_g1(const KParams &P, double x) {
   }
   return pow_general(x, -P.g1);
 }
+template <bool HALLEY = true>
 __device__ __forceinline__ double pow_g1(const KParams &P, double x) {
   const int e = __builtin_amdgcn_frexp_exp(x);
   if (P.g1_third && e > -200 && e < 200) {
     double y, c;
-    fm::third_root(x, e, P.g1_dln2, y, c);
+    fm::third_root<HALLEY>(x, e, P.g1_dln2, y, c);
     const double v = x * y;
     return __builtin_fma(v, fm::expm1_tiny(-c), v);
   }
@@ -76,12 +77,16 @@ __device__ __forceinline__ int q2w(const KParams &P, T q[6], T w[5], T &rinv) {
 // min_d dx_d / (|u_d| + a) taken as 1 / max_d (|u_d| + a) / dx_d, one
 // reciprocal instead of three (equal to a few ulp).
 template <typename T>
-__device__ __forceinline__ double cfl_cell(const KParams &P, const T w[5], T rinv) {
+__device__ __forceinline__ double cfl_speed(const KParams &P, const T w[5], T rinv) {
   const double a = fm::sqrt_pos(P.g * double(w[4]) * double(rinv));
   const double sx = (fabs(double(w[1])) + a) * P.dxinv;
   const double sy = (fabs(double(w[2])) + a) * P.dyinv;
   const double sz = (fabs(double(w[3])) + a) * P.dzinv;
-  return fm::rcp(dmax(dmax(sx, sy), sz));
+  return dmax(dmax(sx, sy), sz);
+}
+template <typename T>
+__device__ __forceinline__ double cfl_cell(const KParams &P, const T w[5], T rinv) {
+  return fm::rcp(cfl_speed(P, w, rinv));
 }
 
 // muscl.f90:100-104 with s(5)/rho as s(5)*rinv
