@@ -58,6 +58,11 @@ struct evh_ctx {
   double *sin2 = nullptr;
   double *rho_part = nullptr;
   void *plan_fwd = nullptr, *plan_inv = nullptr;  // hipfftHandle
+  // power-of-two nz: the plans transform x and y, and the z transform is
+  // the pencil kernel of gravity.hip with these twiddles
+  bool pencils = false;
+  long long spec_plane = 0;  // complex values per z plane of the spectrum
+  void *twiddle = nullptr;   // exp(-2 pi i t / nz), t < nz
   bool phi_valid = false;
   double cosmo = 1.0;  // expansion factor of the last evh_gravpotential
 

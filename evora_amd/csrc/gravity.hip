@@ -3,8 +3,8 @@
 // (gravitation.f90:172-254), one rank.
 //
 // Potential.  The source grav (rho - mean rho) (or, supercomoving,
-// cosmo grav (rho / fb - 1)) goes through a real-to-complex FFT (hipFFT,
-// FP64 in both precision forms), is multiplied by the Green's function of
+// cosmo grav (rho / fb - 1)) goes through a real-to-complex FFT (FP64 in
+// both precision forms), is multiplied by the Green's function of
 // multigreen, -h2 / (sin^2(kfx2 kx) + sin^2(kfy2 ky) + sin^2(kfz2 kz)) (0 at
 // k = 0), and comes back through the complex-to-real FFT.  The reference
 // transforms complex to complex and keeps the real part; since its source is
@@ -15,6 +15,16 @@
 // middle frequencies whose sines differ by the rounding of the
 // single-precision pi of global.f90:98.  The potential stays in FP64 in the
 // mixed form too: the force is a difference of neighbouring potentials.
+//
+// Transforms.  When nz is a power of two, hipFFT transforms x and y (2-D,
+// real to half-complex, one transform per z plane) and the z transform is
+// the pencil kernel below: a block takes a tile of consecutive kx columns of
+// one y row, transforms each column along z with Stockham stages of radix 8
+// (the first reading the spectrum, the others the tile in LDS), applies the
+// Green's function to the last forward stage's outputs, transforms back and
+// writes the tile from the last inverse stage, so that the spectrum is read
+// and written once between the two 2-D transforms instead of three times.
+// Other sizes take hipFFT's 3-D plans and k_green.
 //
 // Force and update.  k_gravint marches z planes of 64 x 4 column tiles with
 // the potential of the tile and its one-cell ring in LDS (periodic wrap, as
@@ -42,7 +52,7 @@ constexpr int GRING = 2 * GX + 2 * GY;
 constexpr int kZChunk = 128;
 
 struct GParams {
-  double grav, fb, h2, n3;
+  double grav, fb, h2, n3, n3inv;
   double dx2inv, dy2inv, dz2inv;
   int supercomoving;
 };
@@ -97,32 +107,223 @@ __global__ void __launch_bounds__(256) k_grav_source(const S *__restrict__ q, lo
   }
 }
 
-// ---- multigreen (gravitation.f90:55-89) on the half spectrum (kx = 0..nx/2)
+// ---- z pencils of the spectrum (power-of-two nz)
+#ifndef EVH_PENCIL_LOG
+#define EVH_PENCIL_LOG 12
+#endif
+constexpr int kPencilLog = EVH_PENCIL_LOG;
+constexpr int kPencilElems = 1 << kPencilLog;  // complex values per block, 16 B each in LDS
+constexpr int kPencilThreads = kPencilElems / 8;  // 8 values per thread and stage
+#ifndef EVH_PENCIL_WAVES
+#define EVH_PENCIL_WAVES 4
+#endif
+constexpr int kPencilWaves = EVH_PENCIL_WAVES;  // waves per SIMD the register budget allows
+
+struct Pencils {
+  int ln;            // log2 of the transform length nz
+  int nxh;           // kx values of a row
+  int ktiles;        // tiles of kx per outer index
+  long long s_elem;  // stride of the transform index, z (complex values)
+  long long s_outer; // stride of the outer index, y
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// The radix-R DFT of v in place, y_s = sum_r v_r w^(r s) with
+// w = exp(SIGN 2 pi i / R): radix 4 as two radix-2 layers, radix 8 as two
+// radix-4 DFTs of the even and odd terms joined by w^s.
+template <int SIGN>
+__device__ __forceinline__ double2 mul_i(double2 d) {  // SIGN i d
+  return SIGN < 0 ? make_double2(d.y, -d.x) : make_double2(-d.y, d.x);
+}
+
+template <int SIGN, int R>
+__device__ __forceinline__ void dft(double2 *v) {
+  if (R == 2) {
+    const double2 a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+  } else if (R == 4) {
+    const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const double2 a2 = cadd(v[1], v[3]), a3 = mul_i<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[2] = csub(a0, a2);
+    v[3] = csub(a1, a3);
+  } else {
+    double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft<SIGN, 4>(e);
+    dft<SIGN, 4>(o);
+    constexpr double c = 0.70710678118654752440;  // 1/sqrt(2)
+    // w = c (1 + SIGN i), w^2 = SIGN i, w^3 = c (-1 + SIGN i)
+    const double2 o1 = make_double2(c * (o[1].x - SIGN * o[1].y), c * (o[1].y + SIGN * o[1].x));
+    const double2 o2 = mul_i<SIGN>(o[2]);
+    const double2 o3 = make_double2(c * (-o[3].x - SIGN * o[3].y), c * (-o[3].y + SIGN * o[3].x));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+}
+
+// where a stage reads and writes: the spectrum in HBM or the tile in LDS,
+// and the last forward stage multiplies its outputs by the Green's function
+enum { kFromLds, kFromSpec };
+enum { kToLds, kToLdsGreen, kToSpec };
+
+struct PencilTile {
+  double2 *X;     // tk columns of n values in LDS, X[i * tk + t]
+  double2 *base;  // the tile's first column at z = 0 in the spectrum
+  int cols, kx0, ky;
+};
+
+// multigreen (gravitation.f90:55-89) at (kx, ky, kz) = (i, j, k): the
+// Green's function (symmetrised, see above) times the value divided by n3,
+// as the complex quotient v / (n3 + 0i) (QUOTIENT) or as the product with
+// 1/n3, which differs from it by an ulp
+template <bool QUOTIENT>
+__device__ __forceinline__ double2 green_apply(double2 v, int i, int j, int k, int nx, int ny, int nz,
+                                               const double *__restrict__ sin2, const GParams &G) {
+  const double *sx = sin2, *sy = sin2 + nx, *sz = sin2 + nx + ny;
+  double green;
+  if (i == 0 && j == 0 && k == 0) {
+    green = 0.0;
+  } else {
+    const int im = i == 0 ? 0 : nx - i, jm = j == 0 ? 0 : ny - j, km = k == 0 ? 0 : nz - k;
+    const double g1 = -G.h2 / (sx[i] + sy[j] + sz[k]);
+    const double g2 = -G.h2 / (sx[im] + sy[jm] + sz[km]);
+    green = g1 == g2 ? g1 : 0.5 * (g1 + g2);
+  }
+  double2 o;
+  if (QUOTIENT) {
+    const double den = G.n3 * G.n3 + 0.0 * 0.0;
+    o.x = (v.x * G.n3 + v.y * 0.0) / den * green;
+    o.y = (v.y * G.n3 - v.x * 0.0) / den * green;
+  } else {
+    const double f = green * G.n3inv;
+    o.x = v.x * f;
+    o.y = v.y * f;
+  }
+  return o;
+}
+
+// One Stockham stage of radix R after ns = 2^LNS on the tk = kPencilElems/n
+// columns of the tile (n = 2^LN), tw[e] = exp(-2 pi i e / n), SIGN -1
+// forward and +1 inverse (unnormalised).  Butterfly j < n/R of column t
+// reads i = j + r n/R (r < R), twiddles it by w^(k r n / (ns R)) with
+// k = j mod ns, and writes its R outputs to (j/ns) ns R + k + s ns; after the
+// last stage that index is the frequency.  Every LDS read of a stage
+// completes (barrier) before its LDS writes, and its writes before the next
+// stage.
+template <int SIGN, int LN, int LNS, int R, int SRC, int DST>
+__device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_elem,
+                                             const double2 *__restrict__ tw, const double *__restrict__ sin2,
+                                             const GParams &G, int nx, int ny, int nz) {
+  constexpr int LR = R == 8 ? 3 : R == 4 ? 2 : 1;
+  constexpr int NB = kPencilElems / (R * kPencilThreads);  // butterflies per thread
+  constexpr int LTK = kPencilLog - LN, TK = 1 << LTK;
+  constexpr int NQ = 1 << (LN - LR), NS = 1 << LNS;
+  constexpr int LSTEP = LN - LNS - LR;  // log2 of n / (ns R)
+  double2 v[NB][R];
+  int dst[NB], tcol[NB];
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int b = threadIdx.x + kPencilThreads * u;
+    const int t = b & (TK - 1), j = b >> LTK, k = j & (NS - 1);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = j + r * NQ;
+      double2 x;
+      if (SRC == kFromSpec) x = t < pt.cols ? pt.base[i * s_elem + t] : make_double2(0.0, 0.0);
+      else x = pt.X[i * TK + t];
+      if (r > 0 && LNS > 0) {  // the first stage's twiddles are 1
+        double2 w = tw[(k * r) << LSTEP];
+        if (SIGN > 0) w.y = -w.y;
+        x = cmul(x, w);
+      }
+      v[u][r] = x;
+    }
+    dst[u] = ((j >> LNS) << (LNS + LR)) + k;
+    tcol[u] = t;
+  }
+  if (SRC == kFromLds && DST != kToSpec) __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    dft<SIGN, R>(v[u]);
+    const int t = tcol[u];
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int i = dst[u] + s * NS;
+      if (DST == kToSpec) {
+        if (t < pt.cols) pt.base[i * s_elem + t] = v[u][s];
+      } else if (DST == kToLdsGreen) {
+        pt.X[i * TK + t] = t < pt.cols ? green_apply<false>(v[u][s], pt.kx0 + t, pt.ky, i, nx, ny, nz, sin2, G)
+                                       : v[u][s];
+      } else {
+        pt.X[i * TK + t] = v[u][s];
+      }
+    }
+  }
+  if (DST != kToSpec) __syncthreads();
+}
+
+// The stages of one transform from ns = 2^LNS on: radix 8 while three or
+// more bits of log2 n remain, then one stage of radix 4 or 2.  The forward
+// transform reads the spectrum in its first stage and applies the Green's
+// function in its last; the inverse writes the spectrum in its last.
+template <int SIGN, int LN, int LNS>
+__device__ __forceinline__ void pencil_stages(const PencilTile &pt, long long s_elem,
+                                              const double2 *__restrict__ tw, const double *__restrict__ sin2,
+                                              const GParams &G, int nx, int ny, int nz) {
+  constexpr int REM = LN - LNS;
+  constexpr int R = REM >= 3 ? 8 : REM == 2 ? 4 : 2;
+  constexpr int LR = R == 8 ? 3 : R == 4 ? 2 : 1;
+  constexpr bool LAST = LNS + LR == LN;
+  constexpr int SRC = SIGN < 0 && LNS == 0 ? kFromSpec : kFromLds;
+  constexpr int DST = LAST ? (SIGN < 0 ? kToLdsGreen : kToSpec) : kToLds;
+  pencil_stage<SIGN, LN, LNS, R, SRC, DST>(pt, s_elem, tw, sin2, G, nx, ny, nz);
+  if constexpr (!LAST) pencil_stages<SIGN, LN, LNS + LR>(pt, s_elem, tw, sin2, G, nx, ny, nz);
+}
+
+// A tile of tk consecutive kx columns of one y row, n = nz = 2^LN: the
+// forward transform along z, the Green's function on the last forward
+// stage's outputs, and the inverse transform.  The first stage reads the
+// spectrum and the last writes it, so the tile passes through LDS between
+// stages only.
+template <int LN>
+__global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
+k_green_pencils(double2 *__restrict__ spec, Pencils pc, const double2 *__restrict__ tw,
+                const double *__restrict__ sin2, GParams G, int nx, int ny, int nz) {
+  __shared__ double2 X[kPencilElems];
+  PencilTile pt;
+  pt.X = X;
+  pt.ky = blockIdx.x / pc.ktiles;
+  pt.kx0 = (blockIdx.x - pt.ky * pc.ktiles) << (kPencilLog - LN);
+  pt.cols = min(1 << (kPencilLog - LN), pc.nxh - pt.kx0);
+  pt.base = spec + pt.ky * pc.s_outer + pt.kx0;
+  pencil_stages<-1, LN, 0>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
+  pencil_stages<1, LN, 0>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
+}
+
+// ---- multigreen on the half spectrum (kx = 0..nx/2) after hipFFT's 3-D transform
 // sin2 holds sin^2(kf2 kvalue) for x (nx), y (ny), z (nz)
 __global__ void __launch_bounds__(256) k_green(double2 *__restrict__ spec, int nx, int ny, int nz,
                                                const double *__restrict__ sin2, GParams G) {
   const int nxh = nx / 2 + 1;
   const long long total = (long long)nxh * ny * nz;
-  const double *sx = sin2, *sy = sin2 + nx, *sz = sin2 + nx + ny;
-  const double den = G.n3 * G.n3 + 0.0 * 0.0;  // phi / dble(n3) as the complex quotient
   for (long long idx = blockIdx.x * 256LL + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
     const int i = (int)(idx % nxh);
     const long long r = idx / nxh;
     const int j = (int)(r % ny), k = (int)(r / ny);
-    double green;
-    if (i == 0 && j == 0 && k == 0) {
-      green = 0.0;
-    } else {
-      const int im = i == 0 ? 0 : nx - i, jm = j == 0 ? 0 : ny - j, km = k == 0 ? 0 : nz - k;
-      const double g1 = -G.h2 / (sx[i] + sy[j] + sz[k]);
-      const double g2 = -G.h2 / (sx[im] + sy[jm] + sz[km]);
-      green = g1 == g2 ? g1 : 0.5 * (g1 + g2);
-    }
-    const double2 v = spec[idx];
-    double2 o;
-    o.x = (v.x * G.n3 + v.y * 0.0) / den * green;
-    o.y = (v.y * G.n3 - v.x * 0.0) / den * green;
-    spec[idx] = o;
+    spec[idx] = green_apply<true>(spec[idx], i, j, k, nx, ny, nz, sin2, G);
   }
 }
 
@@ -219,6 +420,7 @@ GParams make_gparams(const evh_ctx *h) {
   g.fb = c.fb;
   g.h2 = c.h2;
   g.n3 = double((long long)c.nx * c.ny * c.nz);
+  g.n3inv = 1.0 / g.n3;
   g.dx2inv = c.dx2inv;
   g.dy2inv = c.dy2inv;
   g.dz2inv = c.dz2inv;
@@ -246,6 +448,33 @@ double kvalue(int i, int n, int nyq) { return i <= nyq ? double(i - 1) : -double
 
 hipError_t fft_status(hipfftResult r) { return r == HIPFFT_SUCCESS ? hipSuccess : hipErrorUnknown; }
 
+// lengths the pencil kernel takes: powers of two up to a block's tile
+bool pencil_length(int n) { return n >= 2 && n <= kPencilElems && (n & (n - 1)) == 0; }
+
+Pencils make_pencils(int n, int nxh, long long s_elem, long long s_outer) {
+  Pencils p;
+  p.ln = 0;
+  while ((1 << p.ln) < n) p.ln++;
+  const int tk = kPencilElems / n;  // kx columns per block
+  p.nxh = nxh;
+  p.ktiles = (nxh + tk - 1) / tk;
+  p.s_elem = s_elem;
+  p.s_outer = s_outer;
+  return p;
+}
+
+// k_green_pencils instantiated for log2 nz = LN .. kPencilLog
+template <int LN>
+void launch_green_pencils(const Pencils &pz, dim3 grid, hipStream_t st, double2 *spec, const double2 *tw,
+                          const double *sin2, const GParams &gp, int nx, int ny, int nz) {
+  if constexpr (LN <= kPencilLog) {
+    if (pz.ln == LN)
+      hipLaunchKernelGGL(k_green_pencils<LN>, grid, dim3(kPencilThreads), 0, st, spec, pz, tw, sin2, gp, nx, ny, nz);
+    else
+      launch_green_pencils<LN + 1>(pz, grid, st, spec, tw, sin2, gp, nx, ny, nz);
+  }
+}
+
 template <typename S, bool EXACT>
 void launch_gravint_t(evh_ctx *h, double dt) {
   const GGeo g = make_ggeo(h);
@@ -259,7 +488,9 @@ void launch_gravint_t(evh_ctx *h, double dt) {
 hipError_t gravity_init(evh_ctx *h) {
   const evh_config &c = h->cfg;
   const long long n = h->grid.n();
-  const long long nspec = (long long)(c.nx / 2 + 1) * c.ny * c.nz;
+  const bool pencils = pencil_length(c.nz);
+  const long long plane = (long long)(c.nx / 2 + 1) * c.ny;
+  const long long nspec = plane * c.nz;
   hipError_t e;
   if ((e = hipMalloc(&h->phi, sizeof(double) * n)) != hipSuccess) return e;
   if ((e = hipMalloc(&h->spec, sizeof(double2) * nspec)) != hipSuccess) return e;
@@ -278,10 +509,36 @@ hipError_t gravity_init(evh_ctx *h) {
   if ((e = hipMemcpy(h->sin2, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return e;
   hipfftHandle pf = nullptr, pi = nullptr;
-  if ((e = fft_status(hipfftPlan3d(&pf, c.nz, c.ny, c.nx, HIPFFT_D2Z))) != hipSuccess) return e;
-  h->plan_fwd = pf;
-  if ((e = fft_status(hipfftPlan3d(&pi, c.nz, c.ny, c.nx, HIPFFT_Z2D))) != hipSuccess) return e;
-  h->plan_inv = pi;
+  h->pencils = pencils;
+  h->spec_plane = plane;
+  if (h->pencils) {
+    // x and y by hipFFT's 2-D transforms, one per z plane (rows of nx reals to
+    // rows of nx/2+1), and the twiddles of the z pencils, exp(-2 pi i t / nz)
+    // in long double rounded once
+    int len[2] = {c.ny, c.nx}, rem[2] = {c.ny, c.nx}, cem[2] = {c.ny, c.nx / 2 + 1};
+    const int rdist = c.ny * c.nx, cdist = (int)plane;
+    if ((e = fft_status(hipfftPlanMany(&pf, 2, len, rem, 1, rdist, cem, 1, cdist, HIPFFT_D2Z, c.nz))) != hipSuccess)
+      return e;
+    h->plan_fwd = pf;
+    if ((e = fft_status(hipfftPlanMany(&pi, 2, len, cem, 1, cdist, rem, 1, rdist, HIPFFT_Z2D, c.nz))) != hipSuccess)
+      return e;
+    h->plan_inv = pi;
+    std::vector<double2> tw;
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int t = 0; t < c.nz; t++) {
+      const long double a = two_pi * (long double)t / (long double)c.nz;
+      tw.push_back(make_double2((double)std::cos(a), -(double)std::sin(a)));
+    }
+    if ((e = hipMalloc(&h->twiddle, sizeof(double2) * tw.size())) != hipSuccess) return e;
+    h->dev_bytes += (long long)(sizeof(double2) * tw.size());
+    if ((e = hipMemcpy(h->twiddle, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+  } else {
+    if ((e = fft_status(hipfftPlan3d(&pf, c.nz, c.ny, c.nx, HIPFFT_D2Z))) != hipSuccess) return e;
+    h->plan_fwd = pf;
+    if ((e = fft_status(hipfftPlan3d(&pi, c.nz, c.ny, c.nx, HIPFFT_Z2D))) != hipSuccess) return e;
+    h->plan_inv = pi;
+  }
   if ((e = fft_status(hipfftSetStream(pf, h->stream))) != hipSuccess) return e;
   return fft_status(hipfftSetStream(pi, h->stream));
 }
@@ -293,6 +550,8 @@ void gravity_release(evh_ctx *h) {
   if (h->spec) hipFree(h->spec);
   if (h->sin2) hipFree(h->sin2);
   if (h->rho_part) hipFree(h->rho_part);
+  if (h->twiddle) hipFree(h->twiddle);
+  h->twiddle = nullptr;
   h->plan_fwd = h->plan_inv = nullptr;
   h->phi = nullptr;
   h->spec = nullptr;
@@ -324,9 +583,16 @@ hipError_t launch_gravpotential(evh_ctx *h, double cosmo) {
   if ((e = fft_status(hipfftExecD2Z((hipfftHandle)h->plan_fwd, h->phi, (hipfftDoubleComplex *)h->spec))) !=
       hipSuccess)
     return e;
-  const long long nspec = (long long)(h->grid.nx / 2 + 1) * h->grid.ny * h->grid.nz;
-  hipLaunchKernelGGL(k_green, dim3(grid_of(nspec)), dim3(256), 0, h->stream, (double2 *)h->spec, h->grid.nx,
-                     h->grid.ny, h->grid.nz, h->sin2, gp);
+  const int nx = h->grid.nx, ny = h->grid.ny, nz = h->grid.nz, nxh = nx / 2 + 1;
+  if (h->pencils) {
+    const Pencils pz = make_pencils(nz, nxh, h->spec_plane, nxh);
+    launch_green_pencils<1>(pz, dim3(ny * pz.ktiles), h->stream, (double2 *)h->spec, (const double2 *)h->twiddle,
+                            h->sin2, gp, nx, ny, nz);
+  } else {
+    const long long nspec = (long long)nxh * ny * nz;
+    hipLaunchKernelGGL(k_green, dim3(grid_of(nspec)), dim3(256), 0, h->stream, (double2 *)h->spec, nx, ny, nz,
+                       h->sin2, gp);
+  }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return fft_status(hipfftExecZ2D((hipfftHandle)h->plan_inv, (hipfftDoubleComplex *)h->spec, h->phi));
 }
