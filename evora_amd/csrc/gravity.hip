@@ -374,7 +374,21 @@ __global__ void __launch_bounds__(GT) k_gravint(KParams P, GParams Gp, GGeo G, S
   double pm = phi[col + plane * wrap(z0 - 1, G.nz)];
   double pc = phi[col + plane * z0];
   double vmin = 1e6;
+  // qn: q of plane k, loaded one plane ahead (q(6) enters only the
+  // dual-energy branch of the primitive below)
+  S qn[6];
+#pragma unroll
+  for (int m = 0; m < 5; m++) qn[m] = q[m * G.n + col + plane * z0];
+  qn[5] = P.dualenergy ? q[5 * G.n + col + plane * z0] : S(0);
   for (int k = z0; k < z1; k++) {
+    double qc[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) qc[m] = double(qn[m]);
+    if (k + 1 < z1) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) qn[m] = q[m * G.n + col + plane * (k + 1)];
+      if (P.dualenergy) qn[5] = q[5 * G.n + col + plane * (k + 1)];
+    }
     const double pn = phi[col + plane * wrap(k + 1, G.nz)];
     F[(cx + 1) + GRX * (cy + 1)] = pc;
     if (ring) F[(rx + 1) + GRX * (ry + 1)] = phi[rcol + plane * k];
@@ -386,11 +400,6 @@ __global__ void __launch_bounds__(GT) k_gravint(KParams P, GParams Gp, GGeo G, S
       f[1] = G.ny == 1 ? 0.0 : -(F[me + GRX] - F[me - GRX]) * Gp.dy2inv;
       f[2] = G.nz == 1 ? 0.0 : -(pn - pm) * Gp.dz2inv;
       const long long c = col + plane * k;
-      // q(6) enters only the dual-energy branch of the primitive below
-      double qc[6];
-#pragma unroll
-      for (int m = 0; m < 5; m++) qc[m] = double(q[m * G.n + c]);
-      qc[5] = P.dualenergy ? double(q[5 * G.n + c]) : 0.0;
       qc[4] = qc[4] + dt * (qc[1] * f[0] + qc[2] * f[1] + qc[3] * f[2]);
 #pragma unroll
       for (int d = 0; d < 3; d++) qc[1 + d] = qc[1 + d] + dt * qc[0] * f[d];
