@@ -41,12 +41,14 @@ def test_potential_matches_oracle(shape):
     assert rel_linf(phi, ref) <= 1e-12
 
 
-@pytest.mark.parametrize("shape", [(15, 32, 16), (34, 8, 64), (16, 4, 2), (40, 64, 32), (7, 2, 128)])
+@pytest.mark.parametrize("shape", [(15, 32, 16), (34, 8, 64), (16, 4, 2), (40, 64, 32), (7, 2, 128),
+                                   (16, 12, 32), (9, 24, 8), (10, 6, 4)])
 def test_potential_pencil_transforms(shape):
     """A power-of-two nz takes the z-pencil kernel (forward z transform,
     Green's function, inverse z transform in LDS) between hipFFT's 2-D x-y
-    transforms, for odd and even nx and tiles wider than the kx range (nz 2,
-    128); a random density puts signal in every bin."""
+    transforms: odd and even nx, power-of-two and other ny, one-stage z
+    transforms (nz 2, 4) and tiles wider than the kx range; a random density
+    puts signal in every bin."""
     nx, ny, nz = shape
     p = ics.jeans_params(nx, ny=ny, nz=nz, xmax=nx / 16, ymax=ny / 16, zmax=nz / 16)
     d = p.derive()
