@@ -62,6 +62,7 @@ struct evh_ctx {
   // the pencil kernel of gravity.hip with these twiddles
   bool pencils = false;
   long long spec_plane = 0;  // complex values per z plane of the spectrum
+  int spec_pitch = 0;        // complex values per ky row of the spectrum (nx/2+1, padded with pencils)
   void *twiddle = nullptr;   // exp(-2 pi i t / nz), t < nz
   bool phi_valid = false;
   double cosmo = 1.0;  // expansion factor of the last evh_gravpotential

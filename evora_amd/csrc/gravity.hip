@@ -457,6 +457,13 @@ double kvalue(int i, int n, int nyq) { return i <= nyq ? double(i - 1) : -double
 
 hipError_t fft_status(hipfftResult r) { return r == HIPFFT_SUCCESS ? hipSuccess : hipErrorUnknown; }
 
+// With the pencil kernel, each row of the half spectrum (nx/2+1 values) is
+// padded to a multiple of 16 values (256 B), so that the rows, and the
+// pencil tiles of kx columns, start on cache-line boundaries: with the dense
+// odd pitch (257 values at nx = 512) most row segments straddle two lines,
+// and both the pencils and hipFFT's y pass read about twice their bytes.
+constexpr int kSpecAlign = 16;
+
 // lengths the pencil kernel takes: powers of two up to a block's tile
 bool pencil_length(int n) { return n >= 2 && n <= kPencilElems && (n & (n - 1)) == 0; }
 
@@ -498,7 +505,9 @@ hipError_t gravity_init(evh_ctx *h) {
   const evh_config &c = h->cfg;
   const long long n = h->grid.n();
   const bool pencils = pencil_length(c.nz);
-  const long long plane = (long long)(c.nx / 2 + 1) * c.ny;
+  const int nxh = c.nx / 2 + 1;
+  const int pitch = pencils ? (nxh + kSpecAlign - 1) / kSpecAlign * kSpecAlign : nxh;
+  const long long plane = (long long)pitch * c.ny;
   const long long nspec = plane * c.nz;
   hipError_t e;
   if ((e = hipMalloc(&h->phi, sizeof(double) * n)) != hipSuccess) return e;
@@ -520,11 +529,12 @@ hipError_t gravity_init(evh_ctx *h) {
   hipfftHandle pf = nullptr, pi = nullptr;
   h->pencils = pencils;
   h->spec_plane = plane;
+  h->spec_pitch = pitch;
   if (h->pencils) {
     // x and y by hipFFT's 2-D transforms, one per z plane (rows of nx reals to
     // rows of nx/2+1), and the twiddles of the z pencils, exp(-2 pi i t / nz)
     // in long double rounded once
-    int len[2] = {c.ny, c.nx}, rem[2] = {c.ny, c.nx}, cem[2] = {c.ny, c.nx / 2 + 1};
+    int len[2] = {c.ny, c.nx}, rem[2] = {c.ny, c.nx}, cem[2] = {c.ny, pitch};
     const int rdist = c.ny * c.nx, cdist = (int)plane;
     if ((e = fft_status(hipfftPlanMany(&pf, 2, len, rem, 1, rdist, cem, 1, cdist, HIPFFT_D2Z, c.nz))) != hipSuccess)
       return e;
@@ -594,7 +604,7 @@ hipError_t launch_gravpotential(evh_ctx *h, double cosmo) {
     return e;
   const int nx = h->grid.nx, ny = h->grid.ny, nz = h->grid.nz, nxh = nx / 2 + 1;
   if (h->pencils) {
-    const Pencils pz = make_pencils(nz, nxh, h->spec_plane, nxh);
+    const Pencils pz = make_pencils(nz, nxh, h->spec_plane, h->spec_pitch);
     launch_green_pencils<1>(pz, dim3(ny * pz.ktiles), h->stream, (double2 *)h->spec, (const double2 *)h->twiddle,
                             h->sin2, gp, nx, ny, nz);
   } else {
