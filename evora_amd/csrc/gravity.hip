@@ -460,8 +460,8 @@ hipError_t fft_status(hipfftResult r) { return r == HIPFFT_SUCCESS ? hipSuccess 
 // With the pencil kernel, each row of the half spectrum (nx/2+1 values) is
 // padded to a multiple of 16 values (256 B), so that the rows, and the
 // pencil tiles of kx columns, start on cache-line boundaries: with the dense
-// odd pitch (257 values at nx = 512) most row segments straddle two lines,
-// and both the pencils and hipFFT's y pass read about twice their bytes.
+// odd pitch (257 values at nx = 512) most row segments straddle two lines:
+// the pencils read twice their bytes, and hipFFT's y pass slows as well.
 constexpr int kSpecAlign = 16;
 
 // lengths the pencil kernel takes: powers of two up to a block's tile
