@@ -6,7 +6,8 @@ Every translation unit is compiled for gfx950 only.  The staged kernels, the
 exact builds of the fused and conduction kernels, and the gravity kernels are
 compiled without floating-point contraction so that their FP64 results follow
 the reference's operation order; packed FP32 instructions are disabled for
-all device code.  The library links hipFFT (ROCm) for the Poisson solve.
+all device code.  The library links hipFFT (ROCm) for the Poisson solve on
+grids whose sides are not all powers of two.
 """
 from __future__ import annotations
 

@@ -58,12 +58,13 @@ struct evh_ctx {
   double *sin2 = nullptr;
   double *rho_part = nullptr;
   void *plan_fwd = nullptr, *plan_inv = nullptr;  // hipfftHandle
-  // power-of-two nz: the plans transform x and y, and the z transform is
-  // the pencil kernel of gravity.hip with these twiddles
+  // power-of-two nz: the z transform is the pencil kernel of gravity.hip and
+  // the plans transform x and y (2-D, per z plane)
   bool pencils = false;
+  bool xyz_pencils = false;  // power-of-two nx and ny too: no plans, the kernels of gravity.hip transform x and y
   long long spec_plane = 0;  // complex values per z plane of the spectrum
   int spec_pitch = 0;        // complex values per ky row of the spectrum (nx/2+1, padded with pencils)
-  void *twiddle = nullptr;   // exp(-2 pi i t / nz), t < nz
+  void *twiddle = nullptr;   // exp(-2 pi i t / n) for n = nz, then (xyz_pencils) ny and nx
   bool phi_valid = false;
   double cosmo = 1.0;  // expansion factor of the last evh_gravpotential
 

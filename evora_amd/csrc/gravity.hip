@@ -16,15 +16,22 @@
 // single-precision pi of global.f90:98.  The potential stays in FP64 in the
 // mixed form too: the force is a difference of neighbouring potentials.
 //
-// Transforms.  When nz is a power of two, hipFFT transforms x and y (2-D,
-// real to half-complex, one transform per z plane) and the z transform is
-// the pencil kernel below: a block takes a tile of consecutive kx columns of
-// one y row, transforms each column along z with Stockham stages of radix 8
-// (the first reading the spectrum, the others the tile in LDS), applies the
-// Green's function to the last forward stage's outputs, transforms back and
-// writes the tile from the last inverse stage, so that the spectrum is read
-// and written once between the two 2-D transforms instead of three times.
-// Other sizes take hipFFT's 3-D plans and k_green.
+// Transforms.  When nz is a power of two, the z transform is the pencil
+// kernel below: a block takes a tile of consecutive kx columns of one y row,
+// transforms each column along z with Stockham stages of radix 8 (the first
+// reading the spectrum, the others the tile in LDS), applies the Green's
+// function to the last forward stage's outputs, transforms back and writes
+// the tile from the last inverse stage, so that the spectrum is read and
+// written once between the forward and inverse x-y transforms.  When nx and
+// ny are powers of two as well, the x and y transforms are this file's too:
+// k_xfwd forms the source of two real x rows as it loads them, transforms
+// them as one complex row and splits the result into the rows' half
+// spectra; y pencils transform forward, the z pass follows, y pencils
+// transform back, and k_xinv joins two half spectra into one complex row
+// whose inverse transform is the two rows of phi.  No pass of its own forms
+// the source, and the spectrum is read and written once per pass.  With
+// other nx or ny, hipFFT's 2-D x-y transforms (one per z plane) surround the
+// z pass; other nz take hipFFT's 3-D plans and k_green.
 //
 // Force and update.  k_gravint marches z planes of 64 x 4 column tiles with
 // the potential of the tile and its one-cell ring in LDS (periodic wrap, as
@@ -176,14 +183,31 @@ __device__ __forceinline__ void dft(double2 *v) {
 
 // where a stage reads and writes: the spectrum in HBM or the tile in LDS,
 // and the last forward stage multiplies its outputs by the Green's function
-enum { kFromLds, kFromSpec };
-enum { kToLds, kToLdsGreen, kToSpec };
+// kFromSource / kToPhi: the x transform's first stage reads the source of
+// two real rows of rho (one as the real, one as the imaginary part) and its
+// inverse's last stage writes the two real rows of phi; kFromHalves: the
+// inverse x transform's first stage forms that complex row from the two
+// rows' half spectra.
+enum { kFromLds, kFromSpec, kFromSource, kFromHalves };
+enum { kToLds, kToLdsGreen, kToSpec, kToPhi };
 
 struct PencilTile {
   double2 *X;     // tk columns of n values in LDS, X[i * tk + t]
-  double2 *base;  // the tile's first column at z = 0 in the spectrum
-  int cols, kx0, ky;
+  double2 *base;  // the tile's first column at index 0 of the transform axis
+  int cols, kx0;
+  int outer;      // y for the z transform, z for the y transform
+  // x transforms: column t is the row pair (2t, 2t+1) after the tile's first
+  // row; base is that row's half spectrum, rows s_elem values apart
+  const void *rho;    // q(1) of the first row (storage type)
+  double *phi;        // phi of the first row
+  double mean, cosmo_grav;
 };
+
+// the source of gravpotential (gravitation.f90:26-53) of one density, as
+// k_grav_source forms it
+__device__ __forceinline__ double grav_source(double rho, const GParams &G, double mean, double cosmo_grav) {
+  return G.supercomoving ? cosmo_grav * (rho / G.fb - 1.0) : G.grav * (rho - mean);
+}
 
 // multigreen (gravitation.f90:55-89) at (kx, ky, kz) = (i, j, k): the
 // Green's function (symmetrised, see above) times the value divided by n3,
@@ -223,7 +247,22 @@ __device__ __forceinline__ double2 green_apply(double2 v, int i, int j, int k, i
 // last stage that index is the frequency.  Every LDS read of a stage
 // completes (barrier) before its LDS writes, and its writes before the next
 // stage.
-template <int SIGN, int LN, int LNS, int R, int SRC, int DST>
+// The tile in LDS: columns (ROWS false) hold value i of column t at
+// i tk + t, and the lanes of a butterfly index j run over t first; rows
+// (ROWS true, the x transforms, whose rows are contiguous in HBM) hold it at
+// t rp + i + i/8 (rp = n + n/8: one pad value per 8, so that the stride-R
+// writes of a stage fall into different banks), and the lanes run over j
+// first.
+template <int LN, bool ROWS>
+struct TileIndex {
+  static constexpr int N = 1 << LN, LTK = kPencilLog - LN, TK = 1 << LTK;
+  static constexpr int RP = N + N / 8;
+  __device__ static __forceinline__ int at(int i, int t) { return ROWS ? t * RP + i + (i >> 3) : i * TK + t; }
+};
+template <bool ROWS>
+constexpr int tile_elems() { return ROWS ? kPencilElems + kPencilElems / 8 : kPencilElems; }
+
+template <int SIGN, int LN, int LNS, int R, int SRC, int DST, typename S, bool ROWS>
 __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_elem,
                                              const double2 *__restrict__ tw, const double *__restrict__ sin2,
                                              const GParams &G, int nx, int ny, int nz) {
@@ -232,18 +271,44 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
   constexpr int LTK = kPencilLog - LN, TK = 1 << LTK;
   constexpr int NQ = 1 << (LN - LR), NS = 1 << LNS;
   constexpr int LSTEP = LN - LNS - LR;  // log2 of n / (ns R)
+  using TI = TileIndex<LN, ROWS>;
   double2 v[NB][R];
   int dst[NB], tcol[NB];
 #pragma unroll
   for (int u = 0; u < NB; u++) {
     const int b = threadIdx.x + kPencilThreads * u;
-    const int t = b & (TK - 1), j = b >> LTK, k = j & (NS - 1);
+    const int t = ROWS ? b >> (LN - LR) : b & (TK - 1);
+    const int j = ROWS ? b & (NQ - 1) : b >> LTK;
+    const int k = j & (NS - 1);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = j + r * NQ;
       double2 x;
-      if (SRC == kFromSpec) x = t < pt.cols ? pt.base[i * s_elem + t] : make_double2(0.0, 0.0);
-      else x = pt.X[i * TK + t];
+      if (SRC == kFromSpec) {
+        x = t < pt.cols ? pt.base[i * s_elem + t] : make_double2(0.0, 0.0);
+      } else if (SRC == kFromSource) {
+        x = make_double2(0.0, 0.0);
+        if (t < pt.cols) {
+          const S *r0 = (const S *)pt.rho + (long long)(2 * t) * (1 << LN);
+          x.x = grav_source(double(r0[i]), G, pt.mean, pt.cosmo_grav);
+          x.y = grav_source(double(r0[i + (1 << LN)]), G, pt.mean, pt.cosmo_grav);
+        }
+      } else if (SRC == kFromHalves) {
+        // Z(k) = A(k) + i B(k) over the whole row, A and B Hermitian; the
+        // imaginary parts at k = 0 and n/2 are dropped, as a C2R transform does
+        x = make_double2(0.0, 0.0);
+        if (t < pt.cols) {
+          constexpr int N = 1 << LN;
+          const bool lo = i <= N / 2;
+          const double2 *ra = pt.base + (long long)(2 * t) * s_elem + (lo ? i : N - i);
+          double2 a = ra[0], b = ra[s_elem];
+          if (!lo) { a.y = -a.y; b.y = -b.y; }
+          if (i == 0 || i == N / 2) { a.y = 0.0; b.y = 0.0; }
+          x = make_double2(a.x - b.y, a.y + b.x);
+        }
+      } else {
+        x = pt.X[TI::at(i, t)];
+      }
       if (r > 0 && LNS > 0) {  // the first stage's twiddles are 1
         double2 w = tw[(k * r) << LSTEP];
         if (SIGN > 0) w.y = -w.y;
@@ -254,7 +319,10 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
     dst[u] = ((j >> LNS) << (LNS + LR)) + k;
     tcol[u] = t;
   }
-  if (SRC == kFromLds && DST != kToSpec) __syncthreads();
+  // LDS read before LDS write, or (a one-stage transform) the spectrum read
+  // by every lane before any lane overwrites it
+  constexpr bool TO_LDS = DST == kToLds || DST == kToLdsGreen;
+  if ((SRC == kFromLds && TO_LDS) || (SRC == kFromSpec && DST == kToSpec)) __syncthreads();
 #pragma unroll
   for (int u = 0; u < NB; u++) {
     dft<SIGN, R>(v[u]);
@@ -264,22 +332,28 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
       const int i = dst[u] + s * NS;
       if (DST == kToSpec) {
         if (t < pt.cols) pt.base[i * s_elem + t] = v[u][s];
+      } else if (DST == kToPhi) {
+        if (t < pt.cols) {
+          double *p0 = pt.phi + (long long)(2 * t) * (1 << LN);
+          p0[i] = v[u][s].x;
+          p0[i + (1 << LN)] = v[u][s].y;
+        }
       } else if (DST == kToLdsGreen) {
-        pt.X[i * TK + t] = t < pt.cols ? green_apply<false>(v[u][s], pt.kx0 + t, pt.ky, i, nx, ny, nz, sin2, G)
-                                       : v[u][s];
+        pt.X[TI::at(i, t)] = t < pt.cols ? green_apply<false>(v[u][s], pt.kx0 + t, pt.outer, i, nx, ny, nz, sin2, G)
+                                         : v[u][s];
       } else {
-        pt.X[i * TK + t] = v[u][s];
+        pt.X[TI::at(i, t)] = v[u][s];
       }
     }
   }
-  if (DST != kToSpec) __syncthreads();
+  if (TO_LDS) __syncthreads();
 }
 
 // The stages of one transform from ns = 2^LNS on: radix 8 while three or
-// more bits of log2 n remain, then one stage of radix 4 or 2.  The forward
-// transform reads the spectrum in its first stage and applies the Green's
-// function in its last; the inverse writes the spectrum in its last.
-template <int SIGN, int LN, int LNS>
+// more bits of log2 n remain, then one stage of radix 4 or 2.  With
+// FROM_SPEC the first stage reads the spectrum (else the tile in LDS); the
+// last stage writes LAST_DST.
+template <int SIGN, int LN, int LNS, int FIRST_SRC, int LAST_DST, typename S = double, bool ROWS = false>
 __device__ __forceinline__ void pencil_stages(const PencilTile &pt, long long s_elem,
                                               const double2 *__restrict__ tw, const double *__restrict__ sin2,
                                               const GParams &G, int nx, int ny, int nz) {
@@ -287,30 +361,102 @@ __device__ __forceinline__ void pencil_stages(const PencilTile &pt, long long s_
   constexpr int R = REM >= 3 ? 8 : REM == 2 ? 4 : 2;
   constexpr int LR = R == 8 ? 3 : R == 4 ? 2 : 1;
   constexpr bool LAST = LNS + LR == LN;
-  constexpr int SRC = SIGN < 0 && LNS == 0 ? kFromSpec : kFromLds;
-  constexpr int DST = LAST ? (SIGN < 0 ? kToLdsGreen : kToSpec) : kToLds;
-  pencil_stage<SIGN, LN, LNS, R, SRC, DST>(pt, s_elem, tw, sin2, G, nx, ny, nz);
-  if constexpr (!LAST) pencil_stages<SIGN, LN, LNS + LR>(pt, s_elem, tw, sin2, G, nx, ny, nz);
+  constexpr int SRC = LNS == 0 ? FIRST_SRC : kFromLds;
+  constexpr int DST = LAST ? LAST_DST : kToLds;
+  pencil_stage<SIGN, LN, LNS, R, SRC, DST, S, ROWS>(pt, s_elem, tw, sin2, G, nx, ny, nz);
+  if constexpr (!LAST)
+    pencil_stages<SIGN, LN, LNS + LR, FIRST_SRC, LAST_DST, S, ROWS>(pt, s_elem, tw, sin2, G, nx, ny, nz);
 }
 
-// A tile of tk consecutive kx columns of one y row, n = nz = 2^LN: the
-// forward transform along z, the Green's function on the last forward
-// stage's outputs, and the inverse transform.  The first stage reads the
-// spectrum and the last writes it, so the tile passes through LDS between
-// stages only.
-template <int LN>
+// what a pencil launch does: the z transform with the Green's function, or
+// the forward or the inverse y transform
+enum { kPencilGreenZ, kPencilFwdY, kPencilInvY };
+
+// A tile of tk consecutive kx columns at one outer index, n = 2^LN along
+// the transform axis.  kPencilGreenZ: the forward transform along z, the
+// Green's function on the last forward stage's outputs, and the inverse
+// transform; kPencilFwdY / kPencilInvY: one transform along y.  The first
+// stage reads the spectrum and the last writes it, so the tile passes
+// through LDS between stages only.
+template <int LN, int MODE>
 __global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
-k_green_pencils(double2 *__restrict__ spec, Pencils pc, const double2 *__restrict__ tw,
-                const double *__restrict__ sin2, GParams G, int nx, int ny, int nz) {
+k_pencils(double2 *__restrict__ spec, Pencils pc, const double2 *__restrict__ tw,
+          const double *__restrict__ sin2, GParams G, int nx, int ny, int nz) {
   __shared__ double2 X[kPencilElems];
   PencilTile pt;
   pt.X = X;
-  pt.ky = blockIdx.x / pc.ktiles;
-  pt.kx0 = (blockIdx.x - pt.ky * pc.ktiles) << (kPencilLog - LN);
+  pt.outer = blockIdx.x / pc.ktiles;
+  pt.kx0 = (blockIdx.x - pt.outer * pc.ktiles) << (kPencilLog - LN);
   pt.cols = min(1 << (kPencilLog - LN), pc.nxh - pt.kx0);
-  pt.base = spec + pt.ky * pc.s_outer + pt.kx0;
-  pencil_stages<-1, LN, 0>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
-  pencil_stages<1, LN, 0>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
+  pt.base = spec + pt.outer * pc.s_outer + pt.kx0;
+  if (MODE == kPencilGreenZ) {
+    pencil_stages<-1, LN, 0, kFromSpec, kToLdsGreen>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
+    pencil_stages<1, LN, 0, kFromLds, kToSpec>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
+  } else {
+    pencil_stages<MODE == kPencilFwdY ? -1 : 1, LN, 0, kFromSpec, kToSpec>(pt, pc.s_elem, tw, sin2, G, nx, ny, nz);
+  }
+}
+
+// The x transforms, n = nx = 2^LN, on tiles of tk row pairs (rows of nx
+// values in (y, z) order; ny is even, so a pair shares its z plane).  Row
+// pair t is transformed as the complex row z = a + i b, Z = A + i B.
+// k_xfwd: a and b are the source of rows 2t and 2t+1 (formed from rho on
+// the load); after the forward transform, A(k) = (Z(k) + conj Z(n-k)) / 2 and
+// B(k) = (Z(k) - conj Z(n-k)) / 2i, k <= n/2, go to the two rows of the half
+// spectrum.  This is the real-to-complex transform of each row, two rows per
+// complex transform.
+template <typename S, int LN>
+__global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
+k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, long long npairs,
+       const double2 *__restrict__ tw, GParams G, double cosmo_grav, const double *__restrict__ mean_part) {
+  constexpr int N = 1 << LN, TK = 1 << (kPencilLog - LN), NH = N / 2 + 1;
+  using TI = TileIndex<LN, true>;
+  __shared__ double2 X[tile_elems<true>()];
+  PencilTile pt;
+  pt.X = X;
+  const long long p0 = (long long)blockIdx.x * TK;  // first pair of the tile
+  pt.cols = (int)min((long long)TK, npairs - p0);
+  pt.kx0 = 0;
+  pt.outer = 0;
+  pt.base = spec + 2 * p0 * pitch;
+  pt.rho = rho + 2 * p0 * N;
+  pt.phi = nullptr;
+  pt.mean = G.supercomoving ? 0.0 : mean_part[kSumBlocks];
+  pt.cosmo_grav = cosmo_grav;
+  pencil_stages<-1, LN, 0, kFromSource, kToLds, S, true>(pt, pitch, tw, nullptr, G, 0, 0, 0);
+  for (int e = threadIdx.x; e < TK * NH; e += kPencilThreads) {
+    const int t = e / NH, k = e - t * NH;
+    if (t < pt.cols) {
+      const double2 zk = X[TI::at(k, t)], zn = X[TI::at((N - k) & (N - 1), t)];
+      const double2 a = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+      const double2 b = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+      double2 *ra = pt.base + (long long)(2 * t) * pitch + k;
+      ra[0] = a;
+      ra[pitch] = b;
+    }
+  }
+}
+
+// k_xinv: the inverse (unnormalised, as a C2R transform) of the two rows'
+// half spectra A and B as the one complex row A + i B; its real and
+// imaginary parts are phi of rows 2t and 2t+1.
+template <int LN>
+__global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
+k_xinv(const double2 *__restrict__ spec, double *__restrict__ phi, long long pitch, long long npairs,
+       const double2 *__restrict__ tw, GParams G) {
+  constexpr int N = 1 << LN, TK = 1 << (kPencilLog - LN);
+  __shared__ double2 X[tile_elems<true>()];
+  PencilTile pt;
+  pt.X = X;
+  const long long p0 = (long long)blockIdx.x * TK;
+  pt.cols = (int)min((long long)TK, npairs - p0);
+  pt.kx0 = 0;
+  pt.outer = 0;
+  pt.base = const_cast<double2 *>(spec) + 2 * p0 * pitch;
+  pt.rho = nullptr;
+  pt.phi = phi + 2 * p0 * N;
+  pt.mean = pt.cosmo_grav = 0.0;
+  pencil_stages<1, LN, 0, kFromHalves, kToPhi, double, true>(pt, pitch, tw, nullptr, G, 0, 0, 0);
 }
 
 // ---- multigreen on the half spectrum (kx = 0..nx/2) after hipFFT's 3-D transform
@@ -479,15 +625,53 @@ Pencils make_pencils(int n, int nxh, long long s_elem, long long s_outer) {
   return p;
 }
 
-// k_green_pencils instantiated for log2 nz = LN .. kPencilLog
-template <int LN>
-void launch_green_pencils(const Pencils &pz, dim3 grid, hipStream_t st, double2 *spec, const double2 *tw,
-                          const double *sin2, const GParams &gp, int nx, int ny, int nz) {
+// k_pencils instantiated for log2 n = LN .. kPencilLog; one block per tile
+// of kx columns and outer index
+template <int MODE, int LN = 1>
+void launch_pencils(const Pencils &pc, int nouter, hipStream_t st, double2 *spec, const double2 *tw,
+                    const double *sin2, const GParams &gp, int nx, int ny, int nz) {
   if constexpr (LN <= kPencilLog) {
-    if (pz.ln == LN)
-      hipLaunchKernelGGL(k_green_pencils<LN>, grid, dim3(kPencilThreads), 0, st, spec, pz, tw, sin2, gp, nx, ny, nz);
+    if (pc.ln == LN)
+      hipLaunchKernelGGL((k_pencils<LN, MODE>), dim3(nouter * pc.ktiles), dim3(kPencilThreads), 0, st, spec, pc,
+                         tw, sin2, gp, nx, ny, nz);
     else
-      launch_green_pencils<LN + 1>(pz, grid, st, spec, tw, sin2, gp, nx, ny, nz);
+      launch_pencils<MODE, LN + 1>(pc, nouter, st, spec, tw, sin2, gp, nx, ny, nz);
+  }
+}
+
+// k_xfwd / k_xinv instantiated for log2 nx = LN .. kPencilLog; one block per
+// tile of row pairs
+template <typename S, int LN = 1>
+void launch_xfwd(int ln, const S *rho, double2 *spec, long long pitch, long long npairs, hipStream_t st,
+                 const double2 *tw, const GParams &gp, double cosmo_grav, const double *mean_part) {
+  if constexpr (LN <= kPencilLog) {
+    constexpr int TK = 1 << (kPencilLog - LN);
+    if (ln == LN)
+      hipLaunchKernelGGL((k_xfwd<S, LN>), dim3((unsigned)((npairs + TK - 1) / TK)), dim3(kPencilThreads), 0, st,
+                         rho, spec, pitch, npairs, tw, gp, cosmo_grav, mean_part);
+    else
+      launch_xfwd<S, LN + 1>(ln, rho, spec, pitch, npairs, st, tw, gp, cosmo_grav, mean_part);
+  }
+}
+template <int LN = 1>
+void launch_xinv(int ln, const double2 *spec, double *phi, long long pitch, long long npairs, hipStream_t st,
+                 const double2 *tw, const GParams &gp) {
+  if constexpr (LN <= kPencilLog) {
+    constexpr int TK = 1 << (kPencilLog - LN);
+    if (ln == LN)
+      hipLaunchKernelGGL((k_xinv<LN>), dim3((unsigned)((npairs + TK - 1) / TK)), dim3(kPencilThreads), 0, st, spec,
+                         phi, pitch, npairs, tw, gp);
+    else
+      launch_xinv<LN + 1>(ln, spec, phi, pitch, npairs, st, tw, gp);
+  }
+}
+
+// exp(-2 pi i t / n), t < n, in long double rounded once
+void append_twiddles(std::vector<double2> &tw, int n) {
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < n; t++) {
+    const long double a = two_pi * (long double)t / (long double)n;
+    tw.push_back(make_double2((double)std::cos(a), -(double)std::sin(a)));
   }
 }
 
@@ -505,6 +689,7 @@ hipError_t gravity_init(evh_ctx *h) {
   const evh_config &c = h->cfg;
   const long long n = h->grid.n();
   const bool pencils = pencil_length(c.nz);
+  const bool xyz = pencils && pencil_length(c.ny) && pencil_length(c.nx);
   const int nxh = c.nx / 2 + 1;
   const int pitch = pencils ? (nxh + kSpecAlign - 1) / kSpecAlign * kSpecAlign : nxh;
   const long long plane = (long long)pitch * c.ny;
@@ -528,25 +713,31 @@ hipError_t gravity_init(evh_ctx *h) {
     return e;
   hipfftHandle pf = nullptr, pi = nullptr;
   h->pencils = pencils;
+  h->xyz_pencils = xyz;
   h->spec_plane = plane;
   h->spec_pitch = pitch;
   if (h->pencils) {
-    // x and y by hipFFT's 2-D transforms, one per z plane (rows of nx reals to
-    // rows of nx/2+1), and the twiddles of the z pencils, exp(-2 pi i t / nz)
-    // in long double rounded once
-    int len[2] = {c.ny, c.nx}, rem[2] = {c.ny, c.nx}, cem[2] = {c.ny, pitch};
-    const int rdist = c.ny * c.nx, cdist = (int)plane;
-    if ((e = fft_status(hipfftPlanMany(&pf, 2, len, rem, 1, rdist, cem, 1, cdist, HIPFFT_D2Z, c.nz))) != hipSuccess)
-      return e;
-    h->plan_fwd = pf;
-    if ((e = fft_status(hipfftPlanMany(&pi, 2, len, cem, 1, cdist, rem, 1, rdist, HIPFFT_Z2D, c.nz))) != hipSuccess)
-      return e;
-    h->plan_inv = pi;
+    // the x and y transforms: the kernels here when nx and ny are powers of
+    // two too, else hipFFT's 2-D transforms, one per z plane (rows of nx
+    // reals to rows of nx/2+1 values)
+    if (!xyz) {
+      int len[2] = {c.ny, c.nx}, rem[2] = {c.ny, c.nx}, cem[2] = {c.ny, pitch};
+      const int rdist = c.ny * c.nx, cdist = (int)plane;
+      if ((e = fft_status(hipfftPlanMany(&pf, 2, len, rem, 1, rdist, cem, 1, cdist, HIPFFT_D2Z, c.nz))) !=
+          hipSuccess)
+        return e;
+      h->plan_fwd = pf;
+      if ((e = fft_status(hipfftPlanMany(&pi, 2, len, cem, 1, cdist, rem, 1, rdist, HIPFFT_Z2D, c.nz))) !=
+          hipSuccess)
+        return e;
+      h->plan_inv = pi;
+    }
+    // the twiddles of the z transforms, then of y and x
     std::vector<double2> tw;
-    const long double two_pi = 6.283185307179586476925286766559005768L;
-    for (int t = 0; t < c.nz; t++) {
-      const long double a = two_pi * (long double)t / (long double)c.nz;
-      tw.push_back(make_double2((double)std::cos(a), -(double)std::sin(a)));
+    append_twiddles(tw, c.nz);
+    if (xyz) {
+      append_twiddles(tw, c.ny);
+      append_twiddles(tw, c.nx);
     }
     if ((e = hipMalloc(&h->twiddle, sizeof(double2) * tw.size())) != hipSuccess) return e;
     h->dev_bytes += (long long)(sizeof(double2) * tw.size());
@@ -558,8 +749,8 @@ hipError_t gravity_init(evh_ctx *h) {
     if ((e = fft_status(hipfftPlan3d(&pi, c.nz, c.ny, c.nx, HIPFFT_Z2D))) != hipSuccess) return e;
     h->plan_inv = pi;
   }
-  if ((e = fft_status(hipfftSetStream(pf, h->stream))) != hipSuccess) return e;
-  return fft_status(hipfftSetStream(pi, h->stream));
+  if (pf && (e = fft_status(hipfftSetStream(pf, h->stream))) != hipSuccess) return e;
+  return pi ? fft_status(hipfftSetStream(pi, h->stream)) : hipSuccess;
 }
 
 void gravity_release(evh_ctx *h) {
@@ -591,6 +782,28 @@ hipError_t launch_gravpotential(evh_ctx *h, double cosmo) {
     hipLaunchKernelGGL(k_rho_mean, dim3(1), dim3(kSumBlocks), 0, h->stream, h->rho_part, gp.n3);
   }
   const double cosmo_grav = cosmo * gp.grav;
+  const int nx = h->grid.nx, ny = h->grid.ny, nz = h->grid.nz, nxh = nx / 2 + 1;
+  if (h->xyz_pencils) {
+    // source and x transform, y, z with the Green's function, y, x to phi
+    double2 *spec = (double2 *)h->spec;
+    const double2 *twz = (const double2 *)h->twiddle, *twy = twz + nz, *twx = twy + ny;
+    const Pencils pz = make_pencils(nz, nxh, h->spec_plane, h->spec_pitch);
+    const Pencils py = make_pencils(ny, nxh, h->spec_pitch, h->spec_plane);
+    const long long npairs = (long long)ny * nz / 2;
+    int lnx = 0;
+    while ((1 << lnx) < nx) lnx++;
+    if (f32)
+      launch_xfwd<float>(lnx, (const float *)h->qbuf[h->cur], spec, h->spec_pitch, npairs, h->stream, twx, gp,
+                         cosmo_grav, h->rho_part);
+    else
+      launch_xfwd<double>(lnx, (const double *)h->qbuf[h->cur], spec, h->spec_pitch, npairs, h->stream, twx, gp,
+                          cosmo_grav, h->rho_part);
+    launch_pencils<kPencilFwdY>(py, nz, h->stream, spec, twy, h->sin2, gp, nx, ny, nz);
+    launch_pencils<kPencilGreenZ>(pz, ny, h->stream, spec, twz, h->sin2, gp, nx, ny, nz);
+    launch_pencils<kPencilInvY>(py, nz, h->stream, spec, twy, h->sin2, gp, nx, ny, nz);
+    launch_xinv(lnx, spec, h->phi, h->spec_pitch, npairs, h->stream, twx, gp);
+    return hipGetLastError();
+  }
   if (f32)
     hipLaunchKernelGGL(k_grav_source<float>, dim3(grid_of(n)), dim3(256), 0, h->stream,
                        (const float *)h->qbuf[h->cur], n, h->phi, gp, cosmo_grav, h->rho_part);
@@ -602,11 +815,10 @@ hipError_t launch_gravpotential(evh_ctx *h, double cosmo) {
   if ((e = fft_status(hipfftExecD2Z((hipfftHandle)h->plan_fwd, h->phi, (hipfftDoubleComplex *)h->spec))) !=
       hipSuccess)
     return e;
-  const int nx = h->grid.nx, ny = h->grid.ny, nz = h->grid.nz, nxh = nx / 2 + 1;
   if (h->pencils) {
     const Pencils pz = make_pencils(nz, nxh, h->spec_plane, h->spec_pitch);
-    launch_green_pencils<1>(pz, dim3(ny * pz.ktiles), h->stream, (double2 *)h->spec, (const double2 *)h->twiddle,
-                            h->sin2, gp, nx, ny, nz);
+    launch_pencils<kPencilGreenZ>(pz, ny, h->stream, (double2 *)h->spec, (const double2 *)h->twiddle, h->sin2, gp,
+                                  nx, ny, nz);
   } else {
     const long long nspec = (long long)nxh * ny * nz;
     hipLaunchKernelGGL(k_green, dim3(grid_of(nspec)), dim3(256), 0, h->stream, (double2 *)h->spec, nx, ny, nz,

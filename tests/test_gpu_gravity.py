@@ -41,14 +41,18 @@ def test_potential_matches_oracle(shape):
     assert rel_linf(phi, ref) <= 1e-12
 
 
-@pytest.mark.parametrize("shape", [(15, 32, 16), (34, 8, 64), (16, 4, 2), (40, 64, 32), (7, 2, 128),
-                                   (16, 12, 32), (9, 24, 8), (10, 6, 4)])
+@pytest.mark.parametrize("shape", [(15, 32, 16), (34, 8, 64), (40, 64, 32), (7, 2, 128), (16, 12, 32), (9, 24, 8),
+                                   (10, 6, 4), (16, 4, 2), (32, 16, 8), (8, 32, 64), (64, 8, 16), (2, 2, 4),
+                                   (4, 64, 2), (128, 2, 8), (512, 2, 2)])
 def test_potential_pencil_transforms(shape):
     """A power-of-two nz takes the z-pencil kernel (forward z transform,
-    Green's function, inverse z transform in LDS) between hipFFT's 2-D x-y
-    transforms: odd and even nx, power-of-two and other ny, one-stage z
-    transforms (nz 2, 4) and tiles wider than the kx range; a random density
-    puts signal in every bin."""
+    Green's function, inverse z transform in LDS).  With other nx or ny,
+    hipFFT's 2-D x-y transforms surround it: odd and even nx, power-of-two
+    and other ny.  With power-of-two nx and ny as well, the x transforms
+    (the source formed on the load, two real rows per complex transform) and
+    the y transforms are this library's own: nx from 2 to 512, ny from 2 to
+    64, one-stage transforms (n 2, 4) and tiles wider than the kx range.  A
+    random density puts signal in every bin."""
     nx, ny, nz = shape
     p = ics.jeans_params(nx, ny=ny, nz=nz, xmax=nx / 16, ymax=ny / 16, zmax=nz / 16)
     d = p.derive()
@@ -78,8 +82,10 @@ def test_potential_unequal_spacing():
         assert rel_linf(qg[..., c], qo[..., c]) <= 1e-12, c
 
 
-def test_potential_supercomoving_source():
-    p = HydroParams(nx=16, ny=12, nz=10, gravity=True, supercomoving=True, fb=0.17, atomic=True)
+@pytest.mark.parametrize("shape", [(16, 12, 10), (16, 8, 4)])
+def test_potential_supercomoving_source(shape):
+    nx, ny, nz = shape
+    p = HydroParams(nx=nx, ny=ny, nz=nz, gravity=True, supercomoving=True, fb=0.17, atomic=True)
     d = p.derive()
     rng = np.random.default_rng(4)
     q = ics.random_smooth(p, seed=3)
@@ -88,9 +94,11 @@ def test_potential_supercomoving_source():
     assert rel_linf(_potential(d, q, cosmo), O.Oracle(d).gravpotential(q, cosmo)) <= 1e-12
 
 
-def test_potential_fp32_mixed_source():
-    """The mixed form takes rho from its FP32 state; the transforms stay FP64."""
-    p = ics.jeans_params(16)
+@pytest.mark.parametrize("n", [16, 12])
+def test_potential_fp32_mixed_source(n):
+    """The mixed form takes rho from its FP32 state; the transforms stay FP64
+    (16: this library's x transform reads the FP32 density)."""
+    p = ics.jeans_params(n)
     d = p.derive()
     q = ics.jeans(p)
     q32 = q.astype(np.float32).astype(np.float64)
