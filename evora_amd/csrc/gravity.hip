@@ -204,9 +204,10 @@ struct PencilTile {
 };
 
 // the source of gravpotential (gravitation.f90:26-53) of one density, as
-// k_grav_source forms it
+// k_grav_source forms it; SC: supercomoving
+template <bool SC>
 __device__ __forceinline__ double grav_source(double rho, const GParams &G, double mean, double cosmo_grav) {
-  return G.supercomoving ? cosmo_grav * (rho / G.fb - 1.0) : G.grav * (rho - mean);
+  return SC ? cosmo_grav * (rho / G.fb - 1.0) : G.grav * (rho - mean);
 }
 
 // multigreen (gravitation.f90:55-89) at (kx, ky, kz) = (i, j, k): the
@@ -262,7 +263,7 @@ struct TileIndex {
 template <bool ROWS>
 constexpr int tile_elems() { return ROWS ? kPencilElems + kPencilElems / 8 : kPencilElems; }
 
-template <int SIGN, int LN, int LNS, int R, int SRC, int DST, typename S, bool ROWS>
+template <int SIGN, int LN, int LNS, int R, int SRC, int DST, typename S, bool ROWS, bool SC>
 __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_elem,
                                              const double2 *__restrict__ tw, const double *__restrict__ sin2,
                                              const GParams &G, int nx, int ny, int nz) {
@@ -290,8 +291,8 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
         x = make_double2(0.0, 0.0);
         if (t < pt.cols) {
           const S *r0 = (const S *)pt.rho + (long long)(2 * t) * (1 << LN);
-          x.x = grav_source(double(r0[i]), G, pt.mean, pt.cosmo_grav);
-          x.y = grav_source(double(r0[i + (1 << LN)]), G, pt.mean, pt.cosmo_grav);
+          x.x = grav_source<SC>(double(r0[i]), G, pt.mean, pt.cosmo_grav);
+          x.y = grav_source<SC>(double(r0[i + (1 << LN)]), G, pt.mean, pt.cosmo_grav);
         }
       } else if (SRC == kFromHalves) {
         // Z(k) = A(k) + i B(k) over the whole row, A and B Hermitian; the
@@ -353,7 +354,8 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
 // more bits of log2 n remain, then one stage of radix 4 or 2.  With
 // FROM_SPEC the first stage reads the spectrum (else the tile in LDS); the
 // last stage writes LAST_DST.
-template <int SIGN, int LN, int LNS, int FIRST_SRC, int LAST_DST, typename S = double, bool ROWS = false>
+template <int SIGN, int LN, int LNS, int FIRST_SRC, int LAST_DST, typename S = double, bool ROWS = false,
+          bool SC = false>
 __device__ __forceinline__ void pencil_stages(const PencilTile &pt, long long s_elem,
                                               const double2 *__restrict__ tw, const double *__restrict__ sin2,
                                               const GParams &G, int nx, int ny, int nz) {
@@ -363,9 +365,9 @@ __device__ __forceinline__ void pencil_stages(const PencilTile &pt, long long s_
   constexpr bool LAST = LNS + LR == LN;
   constexpr int SRC = LNS == 0 ? FIRST_SRC : kFromLds;
   constexpr int DST = LAST ? LAST_DST : kToLds;
-  pencil_stage<SIGN, LN, LNS, R, SRC, DST, S, ROWS>(pt, s_elem, tw, sin2, G, nx, ny, nz);
+  pencil_stage<SIGN, LN, LNS, R, SRC, DST, S, ROWS, SC>(pt, s_elem, tw, sin2, G, nx, ny, nz);
   if constexpr (!LAST)
-    pencil_stages<SIGN, LN, LNS + LR, FIRST_SRC, LAST_DST, S, ROWS>(pt, s_elem, tw, sin2, G, nx, ny, nz);
+    pencil_stages<SIGN, LN, LNS + LR, FIRST_SRC, LAST_DST, S, ROWS, SC>(pt, s_elem, tw, sin2, G, nx, ny, nz);
 }
 
 // what a pencil launch does: the z transform with the Green's function, or
@@ -404,8 +406,8 @@ k_pencils(double2 *__restrict__ spec, Pencils pc, const double2 *__restrict__ tw
 // the load); after the forward transform, A(k) = (Z(k) + conj Z(n-k)) / 2 and
 // B(k) = (Z(k) - conj Z(n-k)) / 2i, k <= n/2, go to the two rows of the half
 // spectrum.  This is the real-to-complex transform of each row, two rows per
-// complex transform.
-template <typename S, int LN>
+// complex transform.  SC: the supercomoving source.
+template <typename S, int LN, bool SC>
 __global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
 k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, long long npairs,
        const double2 *__restrict__ tw, GParams G, double cosmo_grav, const double *__restrict__ mean_part) {
@@ -421,9 +423,9 @@ k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, l
   pt.base = spec + 2 * p0 * pitch;
   pt.rho = rho + 2 * p0 * N;
   pt.phi = nullptr;
-  pt.mean = G.supercomoving ? 0.0 : mean_part[kSumBlocks];
+  pt.mean = SC ? 0.0 : mean_part[kSumBlocks];
   pt.cosmo_grav = cosmo_grav;
-  pencil_stages<-1, LN, 0, kFromSource, kToLds, S, true>(pt, pitch, tw, nullptr, G, 0, 0, 0);
+  pencil_stages<-1, LN, 0, kFromSource, kToLds, S, true, SC>(pt, pitch, tw, nullptr, G, 0, 0, 0);
   for (int e = threadIdx.x; e < TK * NH; e += kPencilThreads) {
     const int t = e / NH, k = e - t * NH;
     if (t < pt.cols) {
@@ -646,9 +648,13 @@ void launch_xfwd(int ln, const S *rho, double2 *spec, long long pitch, long long
                  const double2 *tw, const GParams &gp, double cosmo_grav, const double *mean_part) {
   if constexpr (LN <= kPencilLog) {
     constexpr int TK = 1 << (kPencilLog - LN);
-    if (ln == LN)
-      hipLaunchKernelGGL((k_xfwd<S, LN>), dim3((unsigned)((npairs + TK - 1) / TK)), dim3(kPencilThreads), 0, st,
-                         rho, spec, pitch, npairs, tw, gp, cosmo_grav, mean_part);
+    const dim3 grid((unsigned)((npairs + TK - 1) / TK));
+    if (ln == LN && gp.supercomoving)
+      hipLaunchKernelGGL((k_xfwd<S, LN, true>), grid, dim3(kPencilThreads), 0, st, rho, spec, pitch, npairs, tw, gp,
+                         cosmo_grav, mean_part);
+    else if (ln == LN)
+      hipLaunchKernelGGL((k_xfwd<S, LN, false>), grid, dim3(kPencilThreads), 0, st, rho, spec, pitch, npairs, tw, gp,
+                         cosmo_grav, mean_part);
     else
       launch_xfwd<S, LN + 1>(ln, rho, spec, pitch, npairs, st, tw, gp, cosmo_grav, mean_part);
   }
