@@ -76,14 +76,38 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 template <typename S>
 __global__ void __launch_bounds__(256) k_rho_partials(const S *__restrict__ q, long long n, double *part) {
   __shared__ double red[256];
-  double s = 0.0, c = 0.0;
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-    const double y = double(q[i]) - c;
-    const double t = s + y;
-    c = (t - s) - y;
-    s = t;
+  // four compensated sums over four interleaved streams, so that four loads
+  // are in flight per thread; then their compensated total
+  double s[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long g = (long long)gridDim.x * 256;
+  long long i = blockIdx.x * 256LL + threadIdx.x;
+  for (; i + 3 * g < n; i += 4 * g) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = double(q[i + u * g]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double y = v[u] - c[u];
+      const double t = s[u] + y;
+      c[u] = (t - s[u]) - y;
+      s[u] = t;
+    }
   }
-  red[threadIdx.x] = s;
+  for (; i < n; i += g) {
+    const double y = double(q[i]) - c[0];
+    const double t = s[0] + y;
+    c[0] = (t - s[0]) - y;
+    s[0] = t;
+  }
+  double tot = 0.0, ct = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const double y = (s[u] - c[u]) - ct;
+    const double t = tot + y;
+    ct = (t - tot) - y;
+    tot = t;
+  }
+  red[threadIdx.x] = tot;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
