@@ -151,11 +151,11 @@ constexpr int kPencilThreads = kPencilElems / 8;  // 8 values per thread and sta
 constexpr int kPencilWaves = EVH_PENCIL_WAVES;  // waves per SIMD the register budget allows
 
 struct Pencils {
-  int ln;            // log2 of the transform length nz
+  int ln;            // log2 of the transform length (nz or ny)
   int nxh;           // kx values of a row
   int ktiles;        // tiles of kx per outer index
-  long long s_elem;  // stride of the transform index, z (complex values)
-  long long s_outer; // stride of the outer index, y
+  long long s_elem;  // stride of the transform index, z or y (complex values)
+  long long s_outer; // stride of the outer index, y for the z transform, z for the y transform
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -478,7 +478,7 @@ k_xinv(const double2 *__restrict__ spec, double *__restrict__ phi, long long pit
   pt.cols = (int)min((long long)TK, npairs - p0);
   pt.kx0 = 0;
   pt.outer = 0;
-  pt.base = const_cast<double2 *>(spec) + 2 * p0 * pitch;
+  pt.base = const_cast<double2 *>(spec) + 2 * p0 * pitch;  // read only: the last stage writes phi
   pt.rho = nullptr;
   pt.phi = phi + 2 * p0 * N;
   pt.mean = pt.cosmo_grav = 0.0;
