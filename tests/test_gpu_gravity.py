@@ -43,15 +43,17 @@ def test_potential_matches_oracle(shape):
 
 @pytest.mark.parametrize("shape", [(15, 32, 16), (34, 8, 64), (40, 64, 32), (7, 2, 128), (16, 12, 32), (9, 24, 8),
                                    (10, 6, 4), (16, 4, 2), (32, 16, 8), (8, 32, 64), (64, 8, 16), (2, 2, 4),
-                                   (4, 64, 2), (128, 2, 8), (512, 2, 2)])
+                                   (4, 64, 2), (128, 2, 8), (512, 2, 2), (1024, 2, 4), (4096, 2, 2), (4, 1024, 2),
+                                   (2, 4096, 2), (2, 2, 4096)])
 def test_potential_pencil_transforms(shape):
     """A power-of-two nz takes the z-pencil kernel (forward z transform,
     Green's function, inverse z transform in LDS).  With other nx or ny,
     hipFFT's 2-D x-y transforms surround it: odd and even nx, power-of-two
     and other ny.  With power-of-two nx and ny as well, the x transforms
     (the source formed on the load, two real rows per complex transform) and
-    the y transforms are this library's own: nx from 2 to 512, ny from 2 to
-    64, one-stage transforms (n 2, 4) and tiles wider than the kx range.  A
+    the y transforms are this library's own: nx, ny and nz from 2 to 4096
+    (the longest transform a tile holds: one row pair or column per block),
+    one-stage transforms (n 2, 4) and tiles wider than the kx range.  A
     random density puts signal in every bin."""
     nx, ny, nz = shape
     p = ics.jeans_params(nx, ny=ny, nz=nz, xmax=nx / 16, ymax=ny / 16, zmax=nz / 16)
