@@ -356,12 +356,16 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
     for (int s = 0; s < R; s++) {
       const int i = dst[u] + s * NS;
       if (DST == kToSpec) {
-        if (t < pt.cols) pt.base[i * s_elem + t] = v[u][s];
+        if (t < pt.cols) {
+          double2 *o = pt.base + i * s_elem + t;  // streamed: no later read in this pass
+          __builtin_nontemporal_store(v[u][s].x, &o->x);
+          __builtin_nontemporal_store(v[u][s].y, &o->y);
+        }
       } else if (DST == kToPhi) {
         if (t < pt.cols) {
           double *p0 = pt.phi + (long long)(2 * t) * (1 << LN);
-          p0[i] = v[u][s].x;
-          p0[i + (1 << LN)] = v[u][s].y;
+          __builtin_nontemporal_store(v[u][s].x, p0 + i);
+          __builtin_nontemporal_store(v[u][s].y, p0 + i + (1 << LN));
         }
       } else if (DST == kToLdsGreen) {
         pt.X[TI::at(i, t)] = t < pt.cols ? green_apply<false>(v[u][s], pt.kx0 + t, pt.outer, i, nx, ny, nz, sin2, G)
@@ -457,8 +461,10 @@ k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, l
       const double2 a = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
       const double2 b = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
       double2 *ra = pt.base + (long long)(2 * t) * pitch + k;
-      ra[0] = a;
-      ra[pitch] = b;
+      __builtin_nontemporal_store(a.x, &ra[0].x);
+      __builtin_nontemporal_store(a.y, &ra[0].y);
+      __builtin_nontemporal_store(b.x, &ra[pitch].x);
+      __builtin_nontemporal_store(b.y, &ra[pitch].y);
     }
   }
 }
