@@ -10,8 +10,12 @@
 //                  work queues (with stealing) and run the tile march of
 //                  fused_core.h on them; each box's CFL minimum for the next
 //                  step is reduced into its own status block.
-// No host synchronisation is needed between steps, so the pair is captured in
-// a hipGraph (two steps, one per ping-pong direction) and replayed.  Launch
+// Each box ping-pongs between the two batch buffers on its own: side[b] names
+// the buffer that holds its current state, and flips only on a step the box
+// takes, so a box stopped by tend or by an error keeps its last state while
+// the others run on.  No host synchronisation is needed between steps, and
+// the kernels read the sides from device memory, so one step is captured in
+// a hipGraph once and replayed.  Launch
 // shape (tile height, occupancy target, z chunk, blocks per CU) is picked by
 // an autotuner that times candidates on a snapshot of the state and logs them.
 #include <hip/hip_runtime.h>
@@ -50,6 +54,7 @@ struct Queues {
 struct BoxState {
   double *t, *tend, *dt, *cfl;
   int *go, *active, *steps, *err_step;
+  int *side;   // buffer (0 or 1) that holds the box's current state
   Status *st;  // per-box status: errors, CFL minimum
 };
 
@@ -76,8 +81,8 @@ __device__ __forceinline__ int box_of(const BoxDev *boxes, int nbox, int item) {
 
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-    k_batch_fused(const BoxDev *__restrict__ boxes, int nbox, const S *__restrict__ qin, S *__restrict__ qout,
-                  BoxState bs, Queues *qs) {
+    k_batch_fused(const BoxDev *__restrict__ boxes, int nbox, S *__restrict__ q0, S *__restrict__ q1, BoxState bs,
+                  Queues *qs) {
   __shared__ core::Smem<T, TY> sm;
   __shared__ int s_item;
   const int xcd = blockIdx.x % kXcds;
@@ -88,9 +93,12 @@ __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu
     if (item < 0) break;  // every queue drained: the whole block leaves
     const int b = box_of(boxes, nbox, item);
     if (bs.active[b]) {
+      // k_batch_dt has flipped the side of every box that steps: the new
+      // state goes to that buffer, the current one is read from the other
       const BoxDev &bx = boxes[b];
-      core::march<T, S, SOLVER, LIMITER, TY, false>(bx.P, bx.G, qin, qout, bx.qoff, b, bs.dt[b], &bs.st[b],
-                                             &bs.st[b].vmin_bits, item - bx.tile0, sm);
+      const bool to1 = bs.side[b] != 0;
+      core::march<T, S, SOLVER, LIMITER, TY, false>(bx.P, bx.G, to1 ? q0 : q1, to1 ? q1 : q0, bx.qoff, b, bs.dt[b],
+                                                    &bs.st[b], &bs.st[b].vmin_bits, item - bx.tile0, sm);
     }
     __syncthreads();  // s_item is rewritten only after every lane has used it
   }
@@ -128,6 +136,7 @@ __global__ void k_batch_dt(int nbox, BoxState bs, Queues *qs) {
   bs.dt[b] = dt;
   bs.steps[b] += 1;
   bs.active[b] = 1;
+  bs.side[b] = 1 - bs.side[b];  // this step writes the other buffer
   s.vmin_bits = (unsigned long long)__double_as_longlong(1e6);  // timestep.f90:35
 }
 
@@ -156,22 +165,22 @@ namespace batch {
 
 // launchers for this translation unit's precision
 template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
-static void launch(const BoxDev *boxes, int nbox, const void *qin, void *qout, const BoxState &bs, Queues *qs,
-                   int blocks, hipStream_t stream) {
+static void launch(const BoxDev *boxes, int nbox, void *q0, void *q1, const BoxState &bs, Queues *qs, int blocks,
+                   hipStream_t stream) {
   hipLaunchKernelGGL((k_batch_fused<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks),
-                     dim3(core::TileShape<TY>::NT), 0, stream, boxes, nbox, (const S *)qin, (S *)qout, bs, qs);
+                     dim3(core::TileShape<TY>::NT), 0, stream, boxes, nbox, (S *)q0, (S *)q1, bs, qs);
 }
 
 template <typename T, typename S, int SOLVER, int LIMITER>
-static void launch_shape(int ty, bool hi, const BoxDev *boxes, int nbox, const void *qin, void *qout,
-                         const BoxState &bs, Queues *qs, int blocks, hipStream_t stream) {
+static void launch_shape(int ty, bool hi, const BoxDev *boxes, int nbox, void *q0, void *q1, const BoxState &bs,
+                         Queues *qs, int blocks, hipStream_t stream) {
   constexpr int LO = sizeof(T) == 8 ? 2 : 3, HI = LO + 1;
   if (ty == 8) {
-    if (hi) launch<T, S, SOLVER, LIMITER, 8, HI>(boxes, nbox, qin, qout, bs, qs, blocks, stream);
-    else launch<T, S, SOLVER, LIMITER, 8, LO>(boxes, nbox, qin, qout, bs, qs, blocks, stream);
+    if (hi) launch<T, S, SOLVER, LIMITER, 8, HI>(boxes, nbox, q0, q1, bs, qs, blocks, stream);
+    else launch<T, S, SOLVER, LIMITER, 8, LO>(boxes, nbox, q0, q1, bs, qs, blocks, stream);
   } else {
-    if (hi) launch<T, S, SOLVER, LIMITER, 12, HI>(boxes, nbox, qin, qout, bs, qs, blocks, stream);
-    else launch<T, S, SOLVER, LIMITER, 12, LO>(boxes, nbox, qin, qout, bs, qs, blocks, stream);
+    if (hi) launch<T, S, SOLVER, LIMITER, 12, HI>(boxes, nbox, q0, q1, bs, qs, blocks, stream);
+    else launch<T, S, SOLVER, LIMITER, 12, LO>(boxes, nbox, q0, q1, bs, qs, blocks, stream);
   }
 }
 
@@ -183,9 +192,9 @@ using TT = float;
 using TT = double;
 #endif
 
-hipError_t EVH_BATCH_LAUNCH(int solver, int limiter, int ty, bool hi, const BoxDev *boxes, int nbox, const void *qin,
-                            void *qout, const BoxState &bs, Queues *qs, int blocks, hipStream_t stream) {
-#define EVH_L(SV, LM) launch_shape<TT, TT, SV, LM>(ty, hi, boxes, nbox, qin, qout, bs, qs, blocks, stream)
+hipError_t EVH_BATCH_LAUNCH(int solver, int limiter, int ty, bool hi, const BoxDev *boxes, int nbox, void *q0,
+                            void *q1, const BoxState &bs, Queues *qs, int blocks, hipStream_t stream) {
+#define EVH_L(SV, LM) launch_shape<TT, TT, SV, LM>(ty, hi, boxes, nbox, q0, q1, bs, qs, blocks, stream)
   if (solver == 1 && limiter == 1) EVH_L(1, 1);
   else if (solver == 1 && limiter == 2) EVH_L(1, 2);
   else if (solver == 1 && limiter == 3) EVH_L(1, 3);
@@ -204,8 +213,8 @@ hipError_t EVH_BATCH_LAUNCH(int solver, int limiter, int ty, bool hi, const BoxD
 #if EVH_FUSED_PREC == 64
 namespace evh {
 namespace batch {
-hipError_t launch_batch_f32(int, int, int, bool, const BoxDev *, int, const void *, void *, const BoxState &, Queues *,
-                            int, hipStream_t);
+hipError_t launch_batch_f32(int, int, int, bool, const BoxDev *, int, void *, void *, const BoxState &, Queues *, int,
+                            hipStream_t);
 }  // namespace batch
 }  // namespace evh
 
@@ -223,7 +232,7 @@ struct evh_batch {
   int device = 0, num_cu = 256;
   hipStream_t stream = nullptr;
   void *qbuf[2] = {nullptr, nullptr};
-  int cur = 0;
+  std::vector<int> side;  // host copy of bs.side, refreshed after every run
   void *snap = nullptr;  // autotuner snapshot of the state
   evh::Xfer xfer;  // host transfers (pinned, double-buffered chunks)
   BoxDev *boxes_d = nullptr;
@@ -233,8 +242,9 @@ struct evh_batch {
   std::vector<char> cfl_needed;
   // launch shape
   int tile_ty = 12, wpe_hi = 1, zchunk = 64, blocks_per_cu = 2;
-  // graphs: gexec[d] advances one step from buffer d to 1-d
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // one step of the ensemble, captured once (the kernels read each box's
+  // buffer side from device memory)
+  hipGraphExec_t gexec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string log;
   int err_code = 0, err_box = -1, err_cell[3] = {0, 0, 0};
@@ -315,39 +325,47 @@ int rebuild_items(evh_batch *b) {
   Queues q{};
   for (int j = 0; j <= kXcds; j++) q.start[j] = (int)((long long)items * j / kXcds);
   BHIP(hipMemcpy(b->qs_d, &q, sizeof(Queues), hipMemcpyHostToDevice));
-  for (auto &g : b->gexec)
-    if (g) {
-      hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  if (b->gexec) {
+    hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
   return EVH_OK;
 }
 
-hipError_t launch_step(evh_batch *b, int from) {
+hipError_t launch_step(evh_batch *b) {
   const int nb = b->nbox;
   hipLaunchKernelGGL(k_batch_dt, dim3((nb + 255) / 256), dim3(256), 0, b->stream, nb, b->bs, b->qs_d);
   const int blocks = b->blocks_per_cu * b->num_cu;
   const evh_config &c = b->cfgs[0];
   if (b->precision == EVH_FP32_MIXED)
-    return launch_batch_f32(c.solver, c.limiter, b->tile_ty, b->wpe_hi != 0, b->boxes_d, nb, b->qbuf[from],
-                            b->qbuf[1 - from], b->bs, b->qs_d, blocks, b->stream);
-  return launch_batch_f64(c.solver, c.limiter, b->tile_ty, b->wpe_hi != 0, b->boxes_d, nb, b->qbuf[from],
-                          b->qbuf[1 - from], b->bs, b->qs_d, blocks, b->stream);
+    return launch_batch_f32(c.solver, c.limiter, b->tile_ty, b->wpe_hi != 0, b->boxes_d, nb, b->qbuf[0], b->qbuf[1],
+                            b->bs, b->qs_d, blocks, b->stream);
+  return launch_batch_f64(c.solver, c.limiter, b->tile_ty, b->wpe_hi != 0, b->boxes_d, nb, b->qbuf[0], b->qbuf[1],
+                          b->bs, b->qs_d, blocks, b->stream);
 }
 
-int capture_graphs(evh_batch *b) {
-  for (int d = 0; d < 2; d++) {
-    if (b->gexec[d]) continue;
-    hipGraph_t g = nullptr;
-    BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t le = launch_step(b, d);
-    hipError_t ce = hipStreamEndCapture(b->stream, &g);
-    if (le != hipSuccess) return bfail(b, EVH_ERR_HIP, hipGetErrorString(le));
-    BHIP(ce);
-    hipError_t ie = hipGraphInstantiate(&b->gexec[d], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    BHIP(ie);
-  }
+int capture_graph(evh_batch *b) {
+  if (b->gexec) return EVH_OK;
+  hipGraph_t g = nullptr;
+  BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = launch_step(b);
+  hipError_t ce = hipStreamEndCapture(b->stream, &g);
+  if (le != hipSuccess) return bfail(b, EVH_ERR_HIP, hipGetErrorString(le));
+  BHIP(ce);
+  hipError_t ie = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  BHIP(ie);
+  return EVH_OK;
+}
+
+// element pointer of box i's current state
+void *box_q(evh_batch *b, int i) {
+  char *base = (char *)b->qbuf[b->side[i]];
+  return base + b->qelem * 6 * b->qoff[i];
+}
+
+int refresh_sides(evh_batch *b) {
+  BHIP(hipMemcpy(b->side.data(), b->bs.side, 4 * b->nbox, hipMemcpyDeviceToHost));
   return EVH_OK;
 }
 
@@ -359,10 +377,10 @@ int prepare(evh_batch *b) {
     const long long n = b->ncell[i];
     if (b->precision == EVH_FP32_MIXED)
       hipLaunchKernelGGL(k_batch_cfl<float>, dim3(gridsz(n)), dim3(256), 0, b->stream, b->kps[i], n,
-                         (const float *)b->qbuf[b->cur] + 6 * b->qoff[i], b->bs.st + i);
+                         (const float *)box_q(b, i), b->bs.st + i);
     else
       hipLaunchKernelGGL(k_batch_cfl<double>, dim3(gridsz(n)), dim3(256), 0, b->stream, b->kps[i], n,
-                         (const double *)b->qbuf[b->cur] + 6 * b->qoff[i], b->bs.st + i);
+                         (const double *)box_q(b, i), b->bs.st + i);
     BHIP(hipGetLastError());
     const int one = 1;
     BHIP(hipMemcpyAsync(b->bs.go + i, &one, sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -394,15 +412,28 @@ int check_boxes(evh_batch *b) {
 
 struct Snapshot {
   std::vector<double> t, tend, dt;
-  std::vector<int> go, steps, err_step;
+  std::vector<int> go, steps, err_step, side;
   std::vector<Status> st;
-  int cur;
 };
+
+// every box's current state into (or out of) the snapshot buffer, at the
+// box's own offset
+int copy_boxes(evh_batch *b, bool save) {
+  for (int i = 0; i < b->nbox; i++) {
+    char *snap = (char *)b->snap + b->qelem * 6 * b->qoff[i];
+    void *cur = box_q(b, i);
+    const size_t bytes = b->qelem * 6 * b->ncell[i];
+    BHIP(hipMemcpyAsync(save ? (void *)snap : cur, save ? cur : (void *)snap, bytes, hipMemcpyDeviceToDevice,
+                        b->stream));
+  }
+  return EVH_OK;
+}
 
 int save_state(evh_batch *b, Snapshot &s) {
   const int nb = b->nbox;
   if (!b->snap) BHIP(hipMalloc(&b->snap, b->qelem * 6 * b->total));
-  BHIP(hipMemcpyAsync(b->snap, b->qbuf[b->cur], b->qelem * 6 * b->total, hipMemcpyDeviceToDevice, b->stream));
+  int r = copy_boxes(b, true);
+  if (r) return r;
   s.t.resize(nb); s.tend.resize(nb); s.dt.resize(nb); s.go.resize(nb); s.steps.resize(nb); s.err_step.resize(nb);
   s.st.resize(nb);
   BHIP(hipMemcpy(s.t.data(), b->bs.t, 8 * nb, hipMemcpyDeviceToHost));
@@ -412,14 +443,16 @@ int save_state(evh_batch *b, Snapshot &s) {
   BHIP(hipMemcpy(s.steps.data(), b->bs.steps, 4 * nb, hipMemcpyDeviceToHost));
   BHIP(hipMemcpy(s.err_step.data(), b->bs.err_step, 4 * nb, hipMemcpyDeviceToHost));
   BHIP(hipMemcpy(s.st.data(), b->bs.st, sizeof(Status) * nb, hipMemcpyDeviceToHost));
-  s.cur = b->cur;
+  s.side = b->side;
   return EVH_OK;
 }
 
 int restore_state(evh_batch *b, const Snapshot &s) {
   const int nb = b->nbox;
-  b->cur = s.cur;
-  BHIP(hipMemcpyAsync(b->qbuf[b->cur], b->snap, b->qelem * 6 * b->total, hipMemcpyDeviceToDevice, b->stream));
+  b->side = s.side;
+  BHIP(hipMemcpy(b->bs.side, s.side.data(), 4 * nb, hipMemcpyHostToDevice));
+  int r = copy_boxes(b, false);
+  if (r) return r;
   BHIP(hipMemcpy(b->bs.t, s.t.data(), 8 * nb, hipMemcpyHostToDevice));
   BHIP(hipMemcpy(b->bs.tend, s.tend.data(), 8 * nb, hipMemcpyHostToDevice));
   BHIP(hipMemcpy(b->bs.dt, s.dt.data(), 8 * nb, hipMemcpyHostToDevice));
@@ -490,8 +523,10 @@ int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
   BoxState &s = b->bs;
   if (hipMalloc(&s.t, 8 * nbox) || hipMalloc(&s.tend, 8 * nbox) || hipMalloc(&s.dt, 8 * nbox) ||
       hipMalloc(&s.cfl, 8 * nbox) || hipMalloc(&s.go, 4 * nbox) || hipMalloc(&s.active, 4 * nbox) ||
-      hipMalloc(&s.steps, 4 * nbox) || hipMalloc(&s.err_step, 4 * nbox) || hipMalloc(&s.st, sizeof(Status) * nbox))
+      hipMalloc(&s.steps, 4 * nbox) || hipMalloc(&s.err_step, 4 * nbox) || hipMalloc(&s.side, 4 * nbox) ||
+      hipMalloc(&s.st, sizeof(Status) * nbox))
     return bail(EVH_ERR_HIP);
+  b->side.assign(nbox, 0);
   std::vector<double> zeros(nbox, 0.0), big(nbox, 1e99), cfl(nbox);
   std::vector<int> izero(nbox, 0), ineg(nbox, -1);
   for (int i = 0; i < nbox; i++) cfl[i] = cfgs[i].cfl;
@@ -508,6 +543,7 @@ int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
       hipMemcpy(s.active, izero.data(), 4 * nbox, hipMemcpyHostToDevice) ||
       hipMemcpy(s.steps, izero.data(), 4 * nbox, hipMemcpyHostToDevice) ||
       hipMemcpy(s.err_step, ineg.data(), 4 * nbox, hipMemcpyHostToDevice) ||
+      hipMemcpy(s.side, izero.data(), 4 * nbox, hipMemcpyHostToDevice) ||
       hipMemcpy(s.st, st.data(), sizeof(Status) * nbox, hipMemcpyHostToDevice))
     return bail(EVH_ERR_HIP);
   if (rebuild_items(b)) return bail(EVH_ERR_HIP);
@@ -518,16 +554,15 @@ int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
 int evh_batch_destroy(evh_batch *b) {
   if (!b) return EVH_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
-  for (auto g : b->gexec)
-    if (g) hipGraphExecDestroy(g);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
   for (auto p : b->qbuf)
     if (p) hipFree(p);
   if (b->snap) hipFree(b->snap);
   b->xfer.release();
   if (b->boxes_d) hipFree(b->boxes_d);
   if (b->qs_d) hipFree(b->qs_d);
-  void *arrs[] = {b->bs.t, b->bs.tend, b->bs.dt, b->bs.cfl, b->bs.go, b->bs.active, b->bs.steps, b->bs.err_step,
-                  b->bs.st};
+  void *arrs[] = {b->bs.t,     b->bs.tend,  b->bs.dt,       b->bs.cfl,  b->bs.go,
+                  b->bs.active, b->bs.steps, b->bs.err_step, b->bs.side, b->bs.st};
   for (auto p : arrs)
     if (p) hipFree(p);
   if (b->ev0) hipEventDestroy(b->ev0);
@@ -541,8 +576,7 @@ int evh_batch_upload_q(evh_batch *b, int box, const double *q) {
   if (!b || !q || box < 0 || box >= b->nbox) return EVH_ERR_STATE;
   const long long n = b->ncell[box];
   const bool f32 = b->precision == EVH_FP32_MIXED;
-  void *dst = f32 ? (void *)((float *)b->qbuf[b->cur] + 6 * b->qoff[box])
-                  : (void *)((double *)b->qbuf[b->cur] + 6 * b->qoff[box]);
+  void *dst = box_q(b, box);
   hipStream_t s = b->stream;
   BHIP(b->xfer.upload(q, n, s, [=](const double *stage, long long cells, long long base) {
     if (f32)
@@ -561,8 +595,7 @@ int evh_batch_download_q(evh_batch *b, int box, double *q) {
   if (!b || !q || box < 0 || box >= b->nbox) return EVH_ERR_STATE;
   const long long n = b->ncell[box];
   const bool f32 = b->precision == EVH_FP32_MIXED;
-  const void *src = f32 ? (const void *)((const float *)b->qbuf[b->cur] + 6 * b->qoff[box])
-                        : (const void *)((const double *)b->qbuf[b->cur] + 6 * b->qoff[box]);
+  const void *src = box_q(b, box);
   hipStream_t s = b->stream;
   BHIP(b->xfer.download(q, n, s, [=](double *stage, long long cells, long long base) {
     if (f32)
@@ -587,16 +620,16 @@ int evh_batch_run(evh_batch *b, int nsteps, int use_graph, float *ms) {
   if (!b || nsteps < 0) return EVH_ERR_STATE;
   int r = prepare(b);
   if (r) return r;
-  if (use_graph && (r = capture_graphs(b))) return r;
+  if (use_graph && (r = capture_graph(b))) return r;
   BHIP(hipEventRecord(b->ev0, b->stream));
   for (int i = 0; i < nsteps; i++) {
-    if (use_graph) BHIP(hipGraphLaunch(b->gexec[b->cur], b->stream));
-    else BHIP(launch_step(b, b->cur));
-    b->cur = 1 - b->cur;
+    if (use_graph) BHIP(hipGraphLaunch(b->gexec, b->stream));
+    else BHIP(launch_step(b));
   }
   BHIP(hipEventRecord(b->ev1, b->stream));
   BHIP(hipEventSynchronize(b->ev1));
   if (ms) BHIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  if ((r = refresh_sides(b))) return r;
   return check_boxes(b);
 }
 

@@ -57,23 +57,48 @@ def test_batch_graph_equals_eager_over_many_steps():
         assert np.array_equal(a, c)
 
 
-def test_batch_tend_and_error_isolation():
+def alone_until(p, q0, tend, precision=hydro.FP64):
+    with hydro.Hydro(p, precision=precision) as h:
+        h.upload_q(q0)
+        t, dt, n = h.run(10000, tend=tend)
+        return h.download_q(), t, n
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.FP32_MIXED])
+def test_batch_tend_and_error_isolation(precision):
+    """Box 1 stops on an error in its first primitive, box 2 at its tend
+    (updatetime's clamp) after a few steps, box 0 runs on.  Every further
+    step of the batch must leave the stopped boxes' states as they were
+    (each box ping-pongs between the buffers on its own), after odd and even
+    numbers of extra steps alike."""
     ps, qs = boxes()
     qs[1] = qs[1].copy()
     qs[1][3, 4, 5, 4] = 0.25 * (qs[1][3, 4, 5, 1:4] ** 2).sum() / qs[1][3, 4, 5, 0]  # E below kinetic
-    with Batch(ps) as b:
+    tend2 = 0.004
+    q2a, t2a, n2a = alone_until(ps[2], qs[2], tend2, precision)
+    assert 1 <= n2a < 4
+    q1_in = None
+    with Batch(ps, precision=precision) as b:
         for i, q in enumerate(qs):
             b.upload_q(i, q)
-        b.set_time(tend=np.array([1e30, 1e30, 0.004]))
-        with pytest.raises(hydro.EvoraError) as ei:
-            b.run(4)
-        assert ei.value.code == 1 and ei.value.box == 1
-        t, dt, steps, go = b.times()
-        assert steps[1] == 0 and go[1] == 0
-        assert t[2] == pytest.approx(0.004, rel=1e-15) and go[2] == 0  # updatetime's clamp ended box 2
-        q0 = b.download_q(0)
-    qa, ta, _ = alone(ps[0], qs[0], 4, hydro.FP64)
-    assert np.array_equal(q0, qa)
+        q1_in = b.download_q(1)  # box 1 as stored (rounded to FP32 in the mixed form)
+        b.set_time(tend=np.array([1e30, 1e30, tend2]))
+        total = 0
+        for extra in (4, 1, 2, 3):
+            with pytest.raises(hydro.EvoraError) as ei:
+                b.run(extra, graph=extra % 2 == 1)
+            total += extra
+            assert ei.value.code == 1 and ei.value.box == 1
+            t, dt, steps, go = b.times()
+            assert steps[1] == 0 and go[1] == 0
+            assert steps[0] == total
+            assert t[2] == t2a and steps[2] == n2a and go[2] == 0  # updatetime's clamp ended box 2
+            assert np.array_equal(b.download_q(2), q2a), total
+            assert np.array_equal(b.download_q(1), q1_in), total
+            if total == 4:
+                q0 = b.download_q(0)
+                qa, ta, _ = alone(ps[0], qs[0], 4, precision)
+                assert np.array_equal(q0, qa)
 
 
 def test_batch_autotune_restores_state():
