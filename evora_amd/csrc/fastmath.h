@@ -26,6 +26,27 @@ __device__ __forceinline__ double rcp(double x) {
   return __builtin_fma(r, e, r);
 }
 
+// The IEEE quotient a/d, rounded to nearest, from y = rcp(d) above or from
+// the correctly rounded 1/d: q = a y, the exact residual r = a - d q (one
+// fma), and the correction q + r y (Markstein's final step).  It is the
+// sequence the compiler emits for a double division, without the operand
+// scaling (v_div_scale) and special-case fix-up (v_div_fixup) that only
+// operands near the ends of the exponent range need, so the result is the
+// IEEE one for normal a, d with |a| > 2^-969 and a normal quotient; a zero
+// a gives +0 (IEEE: the sign of a/d).  One division by d then costs three
+// instructions once y is known, and several divisions share y.
+__device__ __forceinline__ double div_cr(double a, double d, double y) {
+  const double q = a * y;
+  const double r = __builtin_fma(-d, q, a);
+  return __builtin_fma(r, y, q);
+}
+
+// the correctly rounded 1/d (the IEEE 1.0/d for normal d, |d| < 2^1022)
+__device__ __forceinline__ double rcp_cr(double d) {
+  const double y = rcp(d);
+  return __builtin_fma(__builtin_fma(-d, y, 1.0), y, y);
+}
+
 // sqrt of a positive, normal x (Goldschmidt on the rsq estimate)
 __device__ __forceinline__ double sqrt_pos(double x) {
   const double y = __builtin_amdgcn_rsq(x);

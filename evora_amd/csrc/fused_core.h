@@ -112,7 +112,10 @@ struct Prim<float> {
 };
 
 // Arithmetic of the march.  Phys<false> is the fast form (physics_fast.h:
-// refined hardware reciprocals and roots, cube-root powers, FMA contraction).
+// refined hardware reciprocals and roots, cube-root powers, FMA contraction
+// in the Riemann solvers; in FP64 the primitive recovery and the predictor
+// keep the reference's roundings, with IEEE quotients from shared
+// reciprocals).
 // Phys<true> is the reference's own arithmetic (physics.h: IEEE division
 // and square root, the reference's operation order; its translation unit is
 // compiled without FMA contraction), so that every face state, flux and
@@ -482,15 +485,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       Phys<EXACT>::template predictor_dir<0>(P, wc, rc, sx, a1);
       Phys<EXACT>::template predictor_dir<1>(P, wc, rc, sy, a2);
       Phys<EXACT>::template predictor_dir<2>(P, wc, rc, sz, a3);
-      if (EXACT) {  // muscl.f90:106-107
-        const T ix = T(P.dxinv), iy = T(P.dyinv), iz = T(P.dzinv);
-#pragma unroll
-        for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * ix + a2[m] * iy + a3[m] * iz);
-      } else {
-        const T ex = dt2 * T(P.dxinv), ey = dt2 * T(P.dyinv), ez = dt2 * T(P.dzinv);
-#pragma unroll
-        for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey + a3[m] * ez;
-      }
+      fast::evo5(P, dt2, a1, a2, a3, ev);  // muscl.f90:106-107
     } else {
 #pragma unroll
       for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }

@@ -48,6 +48,43 @@ __device__ __forceinline__ float pow_g1(const KParams &P, float x) { return fm::
 
 // comp.f90:26-85 (q2w + pressure); q may be rewritten when dualenergy is on.
 // Returns 1/rho in rinv.
+//
+// FP64: the reference's roundings.  The pressure p = g1 (E - kin) feeds the
+// normal-momentum flux, whose difference across a cell is the whole
+// momentum update wherever the flow along that axis is slow; one ulp of p
+// there is a relative error of ulp(p) / |rho u_d|.  So densinv = 1/rho, the
+// velocities, m2 (no fused multiply-add), kin = 0.5 m2 / rho and
+// ratio = kin/E are formed as the reference forms them, the two divisions
+// by rho sharing one reciprocal (div_cr); only the powers of the dual-energy
+// branches take the fast form.
+__device__ __forceinline__ int q2w(const KParams &P, double q[6], double w[5], double &rinv) {
+#pragma clang fp contract(off)
+  const double y = fm::rcp(q[0]);
+  const double densinv = __builtin_fma(__builtin_fma(-q[0], y, 1.0), y, y);  // 1.0 / q(1)
+  rinv = densinv;
+  w[0] = q[0];
+  w[1] = q[1] * densinv;
+  w[2] = q[2] * densinv;
+  w[3] = q[3] * densinv;
+  const double m2 = (q[1] * q[1]) + (q[2] * q[2]) + (q[3] * q[3]);
+  const double kin = fm::div_cr(0.5 * m2, q[0], y);
+  double pr;
+  if (P.dualenergy) {
+    if (fm::div_cr(kin, q[4], fm::rcp(q[4])) > P.de_thresh1) {  // ratio = kin/E > de_thresh1
+      pr = q[5] * pow_g1(P, q[0]);
+      q[4] = kin + pr * P.g1inv;
+    } else {
+      pr = P.g1 * (q[4] - kin);
+      q[5] = pr * rpow_g1(P, q[0]);
+    }
+  } else {
+    pr = P.g1 * (q[4] - kin);
+  }
+  w[4] = pr;
+  return (pr <= 0.0) ? kErrNegPressure : kOk;
+}
+
+// FP32: refined hardware reciprocal, fused arithmetic
 template <typename T>
 __device__ __forceinline__ int q2w(const KParams &P, T q[6], T w[5], T &rinv) {
   rinv = fm::rcp(q[0]);
@@ -89,13 +126,41 @@ __device__ __forceinline__ double cfl_cell(const KParams &P, const T w[5], T rin
   return fm::rcp(cfl_speed(P, w, rinv));
 }
 
-// muscl.f90:100-104 with s(5)/rho as s(5)*rinv
-template <int D, typename T>
-__device__ __forceinline__ void predictor_dir(const KParams &P, const T w[5], T rinv, const T s[5], T a[5]) {
+// muscl.f90:100-104.  FP64 in the reference's roundings (the predicted
+// pressure enters the face states like the primitive one): s(5)/rho is the
+// IEEE quotient from the correctly rounded rinv; FP32 as s(5)*rinv.
+template <int D>
+__device__ __forceinline__ void predictor_dir(const KParams &P, const double w[5], double rinv, const double s[5],
+                                              double a[5]) {
+#pragma clang fp contract(off)
 #pragma unroll
   for (int i = 0; i < 5; i++) a[i] = w[D + 1] * s[i];
-  a[4] = a[4] + T(P.g) * w[4] * s[D + 1];
+  a[4] = a[4] + P.g * w[4] * s[D + 1];
+  a[D + 1] = a[D + 1] + fm::div_cr(s[4], w[0], rinv);
+}
+template <int D>
+__device__ __forceinline__ void predictor_dir(const KParams &P, const float w[5], float rinv, const float s[5],
+                                              float a[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) a[i] = w[D + 1] * s[i];
+  a[4] = a[4] + float(P.g) * w[4] * s[D + 1];
   a[D + 1] = a[D + 1] + s[4] * rinv;
+}
+
+// evo (muscl.f90:106-107) from the three directions' predictor terms:
+// FP64 as the reference, dt2 (a1 dxinv + a2 dyinv + a3 dzinv) unfused; FP32
+// with dt2 folded into the three factors
+__device__ __forceinline__ void evo5(const KParams &P, double dt2, const double a1[5], const double a2[5],
+                                     const double a3[5], double ev[5]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * P.dxinv + a2[m] * P.dyinv + a3[m] * P.dzinv);
+}
+__device__ __forceinline__ void evo5(const KParams &P, float dt2, const float a1[5], const float a2[5],
+                                     const float a3[5], float ev[5]) {
+  const float ex = dt2 * float(P.dxinv), ey = dt2 * float(P.dyinv), ez = dt2 * float(P.dzinv);
+#pragma unroll
+  for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey + a3[m] * ez;
 }
 
 // muscl.f90:115-169, one component.  minmod and monotonized central in

@@ -52,12 +52,9 @@ def test_kh256_matches_oracle(nsteps):
     CFL 0.8 for the first step, CFL 0.5 (stable) for the 10-step run.
 
     rho v_z of this configuration is the seeded noise alone (|v_z| < 1e-4)
-    while its update takes the difference of two face pressures of 2.5:
-    one ulp of 2.5 is 2e-12 of max |rho v_z|.  The fast form's refined
-    reciprocals flip that ulp in a few faces, so rho v_z is held to the
-    scale of the momentum vector (1e-12 of max |rho v|) and to 1e-11 of its
-    own maximum; every other component to 1e-12 of its own.  The exact form
-    is bit-identical here (test_gpu_exact.py)."""
+    while its update is the difference of two face pressures of 2.5: one
+    ulp of 2.5 in one face is 7e-13 of max |rho v_z|, so this component
+    holds the pressure of every z face to about an ulp of the reference's."""
     p = ics.kh_params(256, cfl=0.8 if nsteps == 1 else 0.5)
     d = p.derive()
     q0 = ics.kelvin_helmholtz(p)
@@ -65,11 +62,8 @@ def test_kh256_matches_oracle(nsteps):
     qo, to = oracle_run(d, q0, nsteps)
     assert tg == pytest.approx(to, rel=1e-12)
     if nsteps == 1:
-        for c in (0, 1, 2, 4, 5):
+        for c in range(6):
             assert rel_linf(qg[..., c], qo[..., c]) <= ONE_STEP_TOL, c
-        assert rel_linf(qg[..., 3], qo[..., 3]) <= 10 * ONE_STEP_TOL
-        mom = np.sqrt((qo[..., 1:4] ** 2).sum(-1)).max()
-        assert np.sqrt(((qg[..., 1:4] - qo[..., 1:4]) ** 2).sum(-1)).max() <= ONE_STEP_TOL * mom
     else:
         for c in range(6):
             assert rel_l1(qg[..., c], qo[..., c]) <= RUN_L1_TOL, c
