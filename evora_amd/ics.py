@@ -15,7 +15,7 @@ import math
 
 import numpy as np
 
-from .params import HydroParams
+from .params import HydroParams, _f32
 
 
 def _centres(d):
@@ -118,6 +118,75 @@ def blast(params: HydroParams, seed=42, e0=1.0, p0=1e-5, r_cells=3.5, noise=0.01
     q = prim_to_q(d, rho, 0.0, 0.0, 0.0, np.full(shape, p0))
     q[..., 4] = np.where(inside, e0 / dv, q[..., 4])
     q[..., 5] = np.where(inside, (e0 * d.g1 / rho ** d.g1) / dv, q[..., 5])
+    return q
+
+
+def blast_reference(params: HydroParams, d0=1.0, p0=1e-5, r0=0.109375, e0=1.0):
+    """ic/blastwave.f90:59-96 as it stands: uniform rho = d0, E = e0/dv in
+    the cells with r < r0 (dv = dx*dy*dz*count), p = p0 elsewhere; the
+    defaults are the blast-wave test's ic_para (bin/test:176-196)."""
+    d = params.derive()
+    x, y, z = _centres(d)
+    shape = (d.nz, d.ny, d.nx)
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2)
+    inside = r < r0
+    dv = d.dx * d.dy * d.dz * int(inside.sum())
+    q = np.zeros(shape + (6,))
+    q[..., 0] = d0
+    q[..., 4] = np.where(inside, e0 / dv, p0 / d.g1)
+    q[..., 5] = np.where(inside, (e0 * d.g1 / d0 ** d.g1) / dv, p0 / d0 ** d.g1)
+    return q
+
+
+def kh_reference(params: HydroParams):
+    """ic/kevinhelmholtz.f90:40-70 as it stands (the 2-D shear layer of
+    para/kevinhelmholtz): rho = 2, u = 0.5 for |y-0.5| <= 0.25, else rho = 1,
+    u = -0.5; v = 0.1 sin(4 pi x) (exp(-(y-.25)^2/.05) + exp(-(y-.75)^2/.05))
+    with the single-precision pi of global.f90:98; p = 2.5."""
+    d = params.derive()
+    x, y, z = _centres(d)
+    shape = (d.nz, d.ny, d.nx)
+    X = np.broadcast_to(x[None, None, :], shape)
+    Y = np.broadcast_to(y[None, :, None], shape)
+    inner = np.abs(Y - 0.5) <= 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    u = np.where(inner, 0.5, -0.5)
+    pi = _f32(3.1415927)
+    v = 0.1 * np.sin(4 * pi * X) * (np.exp(-((Y - 0.25) ** 2) / 0.05) + np.exp(-((Y - 0.75) ** 2) / 0.05))
+    p = 2.5
+    q = np.zeros(shape + (6,))
+    q[..., 0] = rho
+    q[..., 1] = rho * u
+    q[..., 2] = rho * v
+    q[..., 4] = 0.5 * rho * ((u ** 2) + (v ** 2)) + p / (d.g - 1)
+    q[..., 5] = p / (rho ** (d.g - 1))
+    return q
+
+
+def thermaltest_params(**kw) -> HydroParams:
+    """The reference's thermal-conduction test (bin/test:162-173): 100 cells
+    on x in [0, 100], hydro off, simplekappa, c_cond 0.9, gamma 5/3, outflow,
+    t_end 10."""
+    base = dict(nx=100, ny=1, nz=1, xmin=0.0, xmax=100.0, boundtype=1, cfl=0.5, atomic=True, tstart=0.0,
+                tend=10.0, hydro=False, conduction=True, c_cond=_f32(0.9), simplekappa=True)
+    base.update(kw)
+    return HydroParams(**base)
+
+
+def thermaltest(params: HydroParams, templ=400.0, tempr=800.0, dens=1.0):
+    """ic/thermaltest.f90:26-68: rho = 1 at rest, T = 400 for x <= r0 and
+    800 beyond, r0 = (xmax - xmin)/2 (e = T/g1/temp0*rho, q6 = g1 e / rho**g1)."""
+    d = params.derive()
+    x, y, z = _centres(d)
+    shape = (d.nz, d.ny, d.nx)
+    r0 = 0.5 * (params.xmax - params.xmin)
+    el = templ / d.g1 / d.temp0 * dens
+    er = tempr / d.g1 / d.temp0 * dens
+    e = np.broadcast_to(np.where(x[None, None, :] <= r0, el, er), shape)
+    q = np.zeros(shape + (6,))
+    q[..., 0] = dens
+    q[..., 4] = e
+    q[..., 5] = d.g1 * e / (dens ** d.g1)
     return q
 
 

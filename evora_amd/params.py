@@ -49,6 +49,7 @@ class HydroParams:
     simplekappa: bool = False
     gravity: bool = False  # parameter.f90:57
     supercomoving: bool = False  # parameter.f90:61-64
+    hydro: bool = True  # parameter.f90:44: .false. skips hydrostep and sweep (main.f90:119,155)
     omega0: float = 1.0
     fb: float = 1.0
 
@@ -101,6 +102,7 @@ class HydroParams:
             kfx2=kfx * 0.5, kfy2=kfy * 0.5, kfz2=kfz * 0.5,
             nyqx=max(self.nx // 2, 1), nyqy=max(self.ny // 2, 1), nyqz=max(self.nz // 2, 1),
             dx2inv=0.5 / dx, dy2inv=0.5 / dy, dz2inv=0.5 / dz,
+            hydro_off=0 if self.hydro else 1,
         )
 
     def to_namelist(self) -> str:
@@ -112,6 +114,7 @@ class HydroParams:
 
         return (
             "$para\n"
+            f" hydro = {b(self.hydro)},\n"
             f" secondorder = {b(self.secondorder)}, solver = {self.solver}, limiter = {self.limiter},\n"
             f" cfl = {r(self.cfl)}, g = {r(self.g)}, atomic = {b(self.atomic)},\n"
             f" nx = {self.nx}, ny = {self.ny}, nz = {self.nz},\n"
@@ -176,6 +179,7 @@ class Derived:
     dx2inv: float = 0.0
     dy2inv: float = 0.0
     dz2inv: float = 0.0
+    hydro_off: int = 0
 
     @property
     def ncells(self) -> int:

@@ -3,8 +3,8 @@
  *
  * The reference the HIP library is tested against, never the product.  Every function keeps
  * the reference's operand order so that, compiled without FP contraction,
- * it reproduces the compiled Fortran bit for bit (pinned against oracle/_ref
- * by tests/test_oracle_ref.py).  OpenMP only splits independent cells or
+ * it reproduces the compiled Fortran bit for bit (pinned against the fixtures
+ * oracle/_ref/evref writes, by tests/test_oracle_golden.py).  OpenMP only splits independent cells or
  * pencils; per-cell arithmetic does not depend on the thread count.
  */
 #include "evora_oracle.h"
@@ -741,8 +741,10 @@ int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, do
   if (err) return err;
   ev_bound(p, w);
   *dt = 1e99;
-  err = ev_hydrostep(p, w, dt);
-  if (err) return err;
+  if (!p->hydro_off) { /* main.f90:119 */
+    err = ev_hydrostep(p, w, dt);
+    if (err) return err;
+  }
   if (p->conduction) { /* main.f90:121 */
     err = ev_thermalstep(p, q, w, dt);
     if (err) return err;
@@ -756,8 +758,10 @@ int ev_step(const ev_params *p, double *q, double *w, double *evo, double *s, do
     err = ev_compevo(p, w, evo, s, *dt);
     if (err) return err;
   }
-  err = ev_sweep(p, w, q, evo, s, *dt);
-  if (err) return err;
+  if (!p->hydro_off) { /* main.f90:155 */
+    err = ev_sweep(p, w, q, evo, s, *dt);
+    if (err) return err;
+  }
   if (p->gravity) { /* main.f90:157-164, one rank, not comoving (cosmo = 1) */
     double *phi = malloc(sizeof(double) * (size_t)((long)p->nx * p->ny * p->nz));
     if (!phi) return EV_ERR_CONFIG;

@@ -72,6 +72,8 @@ typedef struct {
   double grav, fb, h2, kfx2, kfy2, kfz2;
   int nyqx, nyqy, nyqz;
   double dx2inv, dy2inv, dz2inv;
+  /* hydro = .false. (parameter.f90:44): no hydrostep, no sweep (main.f90:119,155) */
+  int hydro_off;
 } ev_params;
 
 /* cell of the last error: ix, iy, iz (Fortran indices) */

@@ -58,6 +58,7 @@ class EvParams(C.Structure):
         ("kfx2", C.c_double), ("kfy2", C.c_double), ("kfz2", C.c_double),
         ("nyqx", C.c_int), ("nyqy", C.c_int), ("nyqz", C.c_int),
         ("dx2inv", C.c_double), ("dy2inv", C.c_double), ("dz2inv", C.c_double),
+        ("hydro_off", C.c_int),
     ]
 
 

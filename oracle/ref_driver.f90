@@ -9,10 +9,10 @@
 !   ranges      lx=1..nx etc. as decomposition does for mpisize=1
 !               (parallel.f90:33-64)
 !   arrays      q, w, evo, s allocated and zeroed as main.f90:72-77
-!   loop        primitive, bound, dt=1d99, hydrostep, [thermalstep],
-!               updatetime, compevo, sweep, {gravpotential,
+!   loop        primitive, bound, dt=1d99, (hydrostep), [thermalstep],
+!               updatetime, compevo, (sweep), {gravpotential,
 !               gravintegration}, [primitive, bound, thermalconduction]
-!               (main.f90:109-175; [] with conduction, {} with gravity: the
+!               (main.f90:109-175; () with hydro, [] with conduction, {} with gravity: the
 !               reference's potential needs FFTW-2, absent here, so runs
 !               with gravity are for the GPU drop-in, EVORA_BACKEND=hip)
 !
@@ -168,7 +168,7 @@ program evref
      call primitive(w, q)
      call bound(w)
      dt = 1d99
-     call hydrostep(w)
+     if (hydro) call hydrostep(w)              ! main.f90:119
      if (conduction) call thermalstep(q, w)    ! main.f90:121
      call updatetime(go, dump, k4)
      if (secondorder) call compevo(w, evo, s)
@@ -178,7 +178,7 @@ program evref
         write (u) evo
         write (u) s
      endif
-     call sweep(w, q, evo, s)
+     if (hydro) call sweep(w, q, evo, s)       ! main.f90:155
      if (gravity) then                          ! main.f90:157-164
         call gravpotential(q, phic)
         call gravintegration(q, phic, phibuf2, phibuf4)

@@ -43,6 +43,11 @@ def run_fixtures():
 
 
 @pytest.fixture(scope="session")
+def refcase_fixtures():
+    return np.load(os.path.join(GOLDEN, "refcase_fixtures.npz"))
+
+
+@pytest.fixture(scope="session")
 def golden_index():
     import json
     with open(os.path.join(GOLDEN, "index.json")) as fh:

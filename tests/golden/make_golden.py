@@ -1,6 +1,6 @@
 """Write the golden fixtures from the reference's own compiled hot path.
 
-    make -C oracle ref && python tests/golden/make_golden.py [gravity]
+    make -C oracle ref && python tests/golden/make_golden.py [gravity | refcases]
 
 Runs oracle/_ref/evref (the reference Fortran, built in place from the
 reference sources) on small seeded states and stores inputs and outputs as
@@ -158,9 +158,69 @@ def gravity():
     print(f"wrote {len(fx)} gravity arrays")
 
 
+def refcase_list():
+    """The reference's own test cases and parameter files at gamma = 1.4
+    (g = 1.4, atomic = .false., the default of parameter.f90:46), the thermal
+    conduction test, and dual energy with gamma = 1.4."""
+    return {
+        # thesis Sod (bin/test:106-119): 1024 cells, boundtype 3, defaults otherwise
+        "sod1024_g14_b3": (ics.sod_params(1024, boundtype=3, cfl=0.5, atomic=False),
+                           lambda p: ics.sod(p, ur=-0.0), 100000),
+        # blast-wave test (bin/test:176-196, ic/blastwave.f90) to its tend
+        "blast32_g14": (ics.blast_params(32, solver=2, cfl=0.5, atomic=False, tend=0.0508),
+                        ics.blast_reference, 100000),
+        # para/kevinhelmholtz (128 x 128 x 1, MC limiter, periodic), 50 steps
+        "kh2d128_g14": (HydroParams(nx=128, ny=128, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.5,
+                                    atomic=False, tend=12.0), ics.kh_reference, 50),
+        # dual energy with gamma = 1.4: the power of g1 reaches p
+        "cosmo24_g14_de": (ics.cosmo_params(24, atomic=False), ics.cosmo_field, 10),
+        "blast24_g14_de_hll": (ics.blast_params(24, solver=1, cfl=0.4, atomic=False, dualenergy=True,
+                                                de_thresh=0.5, limiter=3), ics.blast, 10),
+        # thermal conduction test (bin/test:162-173, ic/thermaltest.f90), hydro off, to tend
+        "thermal100": (ics.thermaltest_params(), ics.thermaltest, 100000),
+        # conduction on 2-D and 1-D grids with hydro on
+        "cond2d_g14": (ics.conduction_params(24, nz=1, atomic=False), ics.conduction_blob, 5),
+    }
+
+
+def refcases():
+    """Fixtures of refcase_list() (refcase_fixtures.npz), and the point blast
+    at its configured CFL 0.6 with HLLC on the seeded 128^3 state: the
+    reference completes 28 steps and stops with a negative pressure in the
+    29th.  Only its (t, dt) pairs, the error line and a checksum of the
+    initial state are stored (the state is regenerated from its seed)."""
+    import hashlib
+    with open(os.path.join(HERE, "index.json")) as fh:
+        index = json.load(fh)
+    fx = {}
+    for name, (p, gen, nmax) in refcase_list().items():
+        q0 = gen(p)
+        qf, steps, log = O.ref_run(p, q0, nmax)
+        assert qf is not None and "ERROR" not in log, log[-400:]
+        fx[name + "/q0"] = q0
+        fx[name + "/q"] = qf
+        fx[name + "/t_dt"] = np.array(steps)
+        index[name] = dict(kind="refcase", params=p.__dict__, nsteps=len(steps))
+    p = ics.blast_params(128, solver=2, cfl=0.6)
+    q0 = ics.blast(p)
+    qf, steps, log = O.ref_run(p, q0, 40)
+    err = [ln.strip() for ln in log.splitlines() if "ERROR" in ln][:1]
+    assert err, "the reference did not stop"
+    fx["blast128_hllc_cfl06/t_dt"] = np.array(steps)
+    index["blast128_hllc_cfl06"] = dict(kind="abort_big", params=p.__dict__, steps_done=len(steps), error=err,
+                                        q0_sha256=hashlib.sha256(np.ascontiguousarray(q0).tobytes()).hexdigest())
+    np.savez_compressed(os.path.join(HERE, "refcase_fixtures.npz"), **fx)
+    with open(os.path.join(HERE, "index.json"), "w") as fh:
+        json.dump(index, fh, indent=1, sort_keys=True)
+    print(f"wrote {len(fx)} arrays")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["gravity"]:
         gravity()
+    elif sys.argv[1:] == ["refcases"]:
+        refcases()
     else:
         main()
         gravity()
+        refcases()

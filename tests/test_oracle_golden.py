@@ -61,3 +61,48 @@ def test_aborts_like_reference(name, run_fixtures, golden_index):
         orc.run(q, 1, t)
     assert ei.value.code == 1  # comp.f90:74-83
     assert "negative pressure" in e["error"][0]
+
+
+INDEX = __import__("json").load(open(__import__("os").path.join(
+    __import__("os").path.dirname(__file__), "golden", "index.json")))
+REFCASES = sorted(n for n, e in INDEX.items() if e["kind"] == "refcase")
+
+
+@pytest.mark.parametrize("name", REFCASES)
+def test_refcase_bitwise(name, refcase_fixtures, golden_index):
+    """The reference's own test cases (thesis Sod, blast wave, 2-D
+    Kelvin-Helmholtz at gamma = 1.4; the thermal conduction test with hydro
+    off; dual energy at gamma = 1.4; conduction on a 2-D grid), run to
+    their end: every (t, dt) and the final q equal the compiled
+    reference's."""
+    p = params_from_index(golden_index[name])
+    q = refcase_fixtures[name + "/q0"].copy()
+    t_dt = refcase_fixtures[name + "/t_dt"]
+    t, dt, n = O.Oracle(p.derive()).run(q, len(t_dt), p.tstart, p.tend)
+    assert n == len(t_dt)
+    assert (t, dt) == tuple(t_dt[-1])
+    assert np.array_equal(q, refcase_fixtures[name + "/q"])
+
+
+def test_blast128_hllc_cfl06_stops_like_reference(refcase_fixtures, golden_index):
+    """The configured point blast (HLLC, CFL 0.6) on the seeded 128^3 state:
+    the reference completes 28 steps and stops with a negative pressure in
+    the 29th; the oracle takes the same 28 (t, dt) and stops there too."""
+    import hashlib
+
+    from evora_amd import ics
+    e = golden_index["blast128_hllc_cfl06"]
+    p = params_from_index(e)
+    q = ics.blast(p)
+    assert hashlib.sha256(np.ascontiguousarray(q).tobytes()).hexdigest() == e["q0_sha256"]
+    t_dt = refcase_fixtures["blast128_hllc_cfl06/t_dt"]
+    assert len(t_dt) == e["steps_done"] == 28
+    orc = O.Oracle(p.derive())
+    t = 0.0
+    for k in range(28):
+        t, dt, n = orc.run(q, 1, t)
+        assert n == 1 and (t, dt) == tuple(t_dt[k]), k
+    with pytest.raises(O.OracleError) as ei:
+        orc.run(q, 1, t)
+    assert ei.value.code == 1
+    assert "negative pressure" in e["error"][0]

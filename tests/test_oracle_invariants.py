@@ -249,3 +249,26 @@ def test_conduction_refuses_full_kappa():
     q = ics.conduction_blob(p)
     with pytest.raises(O.OracleError):
         O.Oracle(p.derive()).run(q, 1)
+
+
+def test_thermal_test_follows_the_error_function(refcase_fixtures, golden_index):
+    """The reference's conduction test (bin/test:162-173): T = 400 | 800
+    across x = 50 on 100 cells, hydro off, simplekappa (kappa_eff = 1/g1,
+    so dT/dt = d2T/dx2 with rho = temp0 = 1).  bin/test:392 overlays the
+    solution 600 + 200 erf((x - 50) / (2 sqrt(t))) at t = 10; the oracle's
+    run to t = 10 lies within 0.2 K of it everywhere (0.13 K measured, the
+    grid's second-order error) and is antisymmetric about x = 50."""
+    import math
+
+    from helpers import params_from_index
+    p = params_from_index(golden_index["thermal100"])
+    d = p.derive()
+    q = refcase_fixtures["thermal100/q0"].copy()
+    t, dt, n = O.Oracle(d).run(q, 100000, p.tstart, p.tend)
+    assert t == p.tend and n == 49
+    T = d.g1 * q[0, 0, :, 4] / q[0, 0, :, 0] * d.temp0
+    x = d.x0 + np.arange(d.nx) * d.dx
+    exact = np.array([600 + 200 * math.erf((xi - 50) / (2 * math.sqrt(t))) for xi in x])
+    assert np.abs(T - exact).max() <= 0.2
+    assert np.abs(T + T[::-1] - 1200).max() <= 1e-9
+    assert np.array_equal(q, refcase_fixtures["thermal100/q"])
