@@ -377,9 +377,19 @@ def cpu_baseline(n, cores, nsteps=2):
                 return None
             secs.append(s[0])
     slowest = max(secs)
+    try:
+        import psutil
+        physical = psutil.cpu_count(logical=False)
+    except Exception:  # noqa: BLE001 -- the count is informative only
+        physical = None
     return dict(value=n ** 3 * nsteps / slowest, unit="cell-updates/s", cores=cores, kind="reference",
                 sample=f"{nsteps} steps of the {n}^3 HLLC blast, one reference process per z-slab "
-                       f"({cores} slabs, own ghost fill, no halo exchange), slowest slab {slowest:.2f} s")
+                       f"({cores} slabs, own ghost fill, no halo exchange), slowest slab {slowest:.2f} s",
+                host_logical_cpus=os.cpu_count(), host_physical_cores=physical,
+                cpus_usable=len(os.sched_getaffinity(0)),
+                compiler="AMD flang -O3 (gfortran is not in the image)",
+                note=f"{cores} processes: the host-CPU share of one GPU on this machine; the remaining "
+                     "cores of the host were left idle")
 
 
 def copy_probe():

@@ -38,6 +38,7 @@ UNITS = [
     ("conduction.o", "conduction.hip", []),
     ("conductionx.o", "conduction.hip", ["-DEVH_COND_EXACT=1", "-ffp-contract=off"]),
     ("gravity.o", "gravity.hip", ["-ffp-contract=off"]),
+    ("mathcheck.o", "mathcheck.hip", []),
     ("batch64.o", "batch.hip", ["-DEVH_FUSED_PREC=64"]),
     ("batch32.o", "batch.hip", ["-DEVH_FUSED_PREC=32"]),
 ]

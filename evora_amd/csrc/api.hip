@@ -102,34 +102,49 @@ __global__ void k_soa_to_aos(const S *__restrict__ src, double *__restrict__ dst
   }
 }
 
-// compensated per-block sums of q1, q5, q6 (snapshot.f90:169-172)
+// Compensated sums of q1, q5, q6 (snapshot.f90:169-172).  Every addition is
+// an error-free transformation (TwoSum: s + x = t + err exactly), so each
+// thread, the block tree and the host carry a (sum, error) pair and the
+// result is the exact sum rounded once, up to the error of adding the error
+// terms (relative 1e-16 of the magnitudes for n up to 2^40 cells).
 constexpr int kSumBlocks = 1024;
+
+__device__ __host__ __forceinline__ void two_sum(double &s, double &c, double x) {
+  const double t = s + x;
+  const double z = t - s;
+  c += (s - (t - z)) + (x - z);
+  s = t;
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_sums(const S *__restrict__ q, long long n, double *partials) {
-  __shared__ double sh[3][256];
+  __shared__ double sh[6][256];
   double s[3] = {0, 0, 0}, comp[3] = {0, 0, 0};
   const int comps[3] = {0, 4, 5};
   for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
 #pragma unroll
-    for (int j = 0; j < 3; j++) {  // Kahan
-      const double y = double(q[comps[j] * n + c]) - comp[j];
-      const double t = s[j] + y;
-      comp[j] = (t - s[j]) - y;
-      s[j] = t;
-    }
+    for (int j = 0; j < 3; j++) two_sum(s[j], comp[j], double(q[comps[j] * n + c]));
   }
 #pragma unroll
-  for (int j = 0; j < 3; j++) sh[j][threadIdx.x] = s[j] - comp[j];
+  for (int j = 0; j < 3; j++) {
+    sh[j][threadIdx.x] = s[j];
+    sh[3 + j][threadIdx.x] = comp[j];
+  }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w)
 #pragma unroll
-      for (int j = 0; j < 3; j++) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+      for (int j = 0; j < 3; j++) {
+        double a = sh[j][threadIdx.x], e = sh[3 + j][threadIdx.x] + sh[3 + j][threadIdx.x + w];
+        two_sum(a, e, sh[j][threadIdx.x + w]);
+        sh[j][threadIdx.x] = a;
+        sh[3 + j][threadIdx.x] = e;
+      }
     __syncthreads();
   }
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int j = 0; j < 3; j++) partials[3 * blockIdx.x + j] = sh[j][0];
+    for (int j = 0; j < 6; j++) partials[6 * blockIdx.x + j] = sh[j][0];
 }
 
 int grid_of(long long work) {
@@ -207,11 +222,14 @@ int validate(const evh_config *c) {
   if (c->precision == EVH_FP32_MIXED && !fused_supported(*c)) return EVH_ERR_CONFIG;
   // conduction: simplekappa only (the reference's other branch reads an
   // unassigned dtempdx, conduction.f90:134-141), no dual energy (thermalstep
-  // evaluates pressure on q a second time, timestep.f90:87), 3-D grids
-  if (c->conduction && (!c->simplekappa || c->dualenergy || !fused_supported(*c) || !(c->temp0 > 0.0)))
-    return EVH_ERR_CONFIG;
+  // evaluates pressure on q a second time, timestep.f90:87); degenerate
+  // directions are skipped as in conduction.f90:46,72,98
+  if (c->conduction && (!c->simplekappa || c->dualenergy || !(c->temp0 > 0.0))) return EVH_ERR_CONFIG;
+  if (c->conduction && c->precision == EVH_FP32_MIXED && !fused_supported(*c)) return EVH_ERR_CONFIG;
+  // a run without hydro (main.f90:119,155) has only conduction to do
+  if (c->hydro_off && !c->conduction) return EVH_ERR_CONFIG;
   // gravity: 3-D grids (the FFT Poisson solve of gravitation.f90 on one rank)
-  if (c->gravity && (!fused_supported(*c) || !(c->h2 > 0.0) || c->nyqx < 1 || c->nyqy < 1 || c->nyqz < 1 ||
+  if (c->gravity && (!fused_supported(*c) || c->nz < 2 || !(c->h2 > 0.0) || c->nyqx < 1 || c->nyqy < 1 || c->nyqz < 1 ||
                      (c->supercomoving && !(c->fb > 0.0))))
     return EVH_ERR_CONFIG;
   return EVH_OK;
@@ -297,8 +315,8 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
   if ((e = h->xfer.init(n)) != hipSuccess) return bail(e);
   if ((e = hipMalloc(&h->st, sizeof(Status))) != hipSuccess) return bail(e);
   if ((e = hipHostMalloc(&h->st_host, sizeof(Status))) != hipSuccess) return bail(e);
-  if ((e = hipMalloc(&h->partials, sizeof(double) * 3 * kSumBlocks)) != hipSuccess) return bail(e);
-  h->dev_bytes += h->xfer.device_bytes() + (long long)sizeof(Status) + (long long)sizeof(double) * 3 * kSumBlocks;
+  if ((e = hipMalloc(&h->partials, sizeof(double) * 6 * kSumBlocks)) != hipSuccess) return bail(e);
+  h->dev_bytes += h->xfer.device_bytes() + (long long)sizeof(Status) + (long long)sizeof(double) * 6 * kSumBlocks;
   hipLaunchKernelGGL(k_clear_code, dim3(1), dim3(64), 0, h->stream, h->st);
   if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(e);
   *out = h;
@@ -330,7 +348,8 @@ int evh_destroy(evh_ctx *h) {
 int evh_set_mode(evh_ctx *h, int mode) {
   if (!h) return EVH_ERR_STATE;
   if (mode != EVH_MODE_FUSED && mode != EVH_MODE_STAGED && mode != EVH_MODE_FUSED_EXACT) return fail(h, EVH_ERR_CONFIG);
-  if (mode != EVH_MODE_STAGED && !fused_supported(h->cfg)) return fail(h, EVH_ERR_CONFIG, "fused needs a 3-D grid");
+  if (mode != EVH_MODE_STAGED && !fused_supported(h->cfg))
+    return fail(h, EVH_ERR_CONFIG, "fused needs a 2-D or 3-D grid");
   if (mode != EVH_MODE_FUSED && mixed(h)) return fail(h, EVH_ERR_CONFIG, "staged and exact kernels are FP64");
   h->mode = mode;
   h->v_valid = h->tv_valid = h->t_valid = false;  // limits and T are formed in the mode's arithmetic
@@ -556,15 +575,18 @@ int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go) {
   if ((r = evh_primitive(h))) return r;
   if ((r = evh_bound(h))) return r;
   *dt = 1e99;  // main.f90:118
-  if ((r = evh_hydrostep(h, dt))) return r;
+  const bool hydro = !h->cfg.hydro_off;
+  if (hydro && (r = evh_hydrostep(h, dt))) return r;                 // main.f90:119
   if (h->cfg.conduction && (r = evh_thermalstep(h, dt))) return r;  // main.f90:121
   if (tend - *t <= *dt) {  // timestep.f90:163-166
     *dt = tend - *t;
     *go = 0;
   }
   *t = *t + *dt;
-  if (h->cfg.secondorder && (r = evh_compevo(h, *dt))) return r;
-  if ((r = evh_sweep(h, *dt))) return r;
+  if (hydro) {  // compevo feeds only the sweep (main.f90:151,155)
+    if (h->cfg.secondorder && (r = evh_compevo(h, *dt))) return r;
+    if ((r = evh_sweep(h, *dt))) return r;
+  }
   if (h->cfg.gravity) {  // main.f90:157-164
     if ((r = evh_gravpotential(h, h->cosmo))) return r;
     if ((r = evh_gravintegration(h, *dt))) return r;
@@ -597,18 +619,16 @@ int evh_sums(evh_ctx *h, double out[3]) {
     hipLaunchKernelGGL(k_sums<double>, dim3(kSumBlocks), dim3(256), 0, h->stream, (const double *)h->qbuf[h->cur], n,
                        h->partials);
   HIP_TRY(hipGetLastError());
-  double part[3 * kSumBlocks];
+  double part[6 * kSumBlocks];
   HIP_TRY(hipMemcpyAsync(part, h->partials, sizeof part, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   for (int j = 0; j < 3; j++) {
     double s = 0, c = 0;
     for (int b = 0; b < kSumBlocks; b++) {
-      const double y = part[3 * b + j] - c;
-      const double t = s + y;
-      c = (t - s) - y;
-      s = t;
+      two_sum(s, c, part[6 * b + j]);
+      c += part[6 * b + 3 + j];
     }
-    out[j] = s;
+    out[j] = s + c;
   }
   return EVH_OK;
 }

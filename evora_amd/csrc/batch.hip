@@ -253,6 +253,23 @@ struct evh_batch {
 
 namespace {
 
+// what every box of a batch must share and support
+int check_configs(const evh_config *cfgs, int nbox) {
+  const evh_config &c0 = cfgs[0];
+  for (int i = 0; i < nbox; i++) {
+    const evh_config &c = cfgs[i];
+    if (!fused_supported(c) || c.nz < 2) return EVH_ERR_CONFIG;  // 3-D boxes only
+    if (c.conduction || c.gravity || c.hydro_off) return EVH_ERR_CONFIG;  // hydro step only
+    if (c.solver != 1 && c.solver != 2) return EVH_ERR_SOLVER;
+    if (c.limiter < 1 || c.limiter > 3) return EVH_ERR_LIMITER;
+    if (c.solver != c0.solver || c.limiter != c0.limiter || c.precision != c0.precision ||
+        c.secondorder != c0.secondorder || c.dualenergy != c0.dualenergy || c.boundtype < 1 || c.boundtype > 3 ||
+        (c.precision != EVH_FP64 && c.precision != EVH_FP32_MIXED))
+      return EVH_ERR_CONFIG;  // one kernel variant for the whole ensemble
+  }
+  return EVH_OK;
+}
+
 int bfail(evh_batch *b, int code, const char *msg, int box = -1) {
   b->err_code = code;
   b->err_box = box;
@@ -297,33 +314,43 @@ int gridsz(long long work) {
   return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
 }
 
-// descriptors and work queues for the current launch shape
-int rebuild_items(evh_batch *b) {
-  std::vector<BoxDev> h(b->nbox);
+// Descriptors of nbox boxes (box i at element offset 6 qoff[i]) for tile
+// height ty and z chunk zc, and the per-XCD work queues over their tiles;
+// returns the number of work items.
+int describe_boxes(const evh_config *cfgs, const KParams *kps, const long long *ncell, const long long *qoff,
+                   int nbox, int ty, int zc, BoxDev *out, Queues *q) {
   int items = 0;
-  for (int i = 0; i < b->nbox; i++) {
-    const evh_config &c = b->cfgs[i];
+  for (int i = 0; i < nbox; i++) {
+    const evh_config &c = cfgs[i];
     Geo g;
     g.nx = c.nx;
     g.ny = c.ny;
     g.nz = c.nz;
-    g.n = b->ncell[i];
-    g.zchunk = b->zchunk > g.nz ? g.nz : b->zchunk;
+    g.n = ncell[i];
+    g.zchunk = zc > g.nz ? g.nz : zc;
     g.tiles_x = (g.nx + core::TX - 1) / core::TX;
-    g.tiles_y = (g.ny + b->tile_ty - 1) / b->tile_ty;
+    g.tiles_y = (g.ny + ty - 1) / ty;
     g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
     g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
-    h[i].P = b->kps[i];
-    h[i].G = g;
-    h[i].qoff = 6 * b->qoff[i];
-    h[i].tile0 = items;
-    h[i].pad = 0;
+    out[i].P = kps[i];
+    out[i].G = g;
+    out[i].qoff = 6 * qoff[i];
+    out[i].tile0 = items;
+    out[i].pad = 0;
     items += g.ntiles;
   }
-  b->nitems = items;
+  *q = Queues{};
+  for (int j = 0; j <= kXcds; j++) q->start[j] = (int)((long long)items * j / kXcds);
+  return items;
+}
+
+// descriptors and work queues for the current launch shape
+int rebuild_items(evh_batch *b) {
+  std::vector<BoxDev> h(b->nbox);
+  Queues q;
+  b->nitems = describe_boxes(b->cfgs.data(), b->kps.data(), b->ncell.data(), b->qoff.data(), b->nbox, b->tile_ty,
+                             b->zchunk, h.data(), &q);
   BHIP(hipMemcpy(b->boxes_d, h.data(), sizeof(BoxDev) * b->nbox, hipMemcpyHostToDevice));
-  Queues q{};
-  for (int j = 0; j <= kXcds; j++) q.start[j] = (int)((long long)items * j / kXcds);
   BHIP(hipMemcpy(b->qs_d, &q, sizeof(Queues), hipMemcpyHostToDevice));
   if (b->gexec) {
     hipGraphExecDestroy(b->gexec);
@@ -471,17 +498,9 @@ extern "C" {
 int evh_batch_create(const evh_config *cfgs, int nbox, evh_batch **out) {
   if (!cfgs || !out || nbox < 1) return EVH_ERR_CONFIG;
   *out = nullptr;
+  const int v = check_configs(cfgs, nbox);
+  if (v) return v;
   const evh_config &c0 = cfgs[0];
-  for (int i = 0; i < nbox; i++) {
-    const evh_config &c = cfgs[i];
-    if (!fused_supported(c)) return EVH_ERR_CONFIG;  // 3-D boxes only
-    if (c.conduction || c.gravity) return EVH_ERR_CONFIG;  // hydro step only
-    if (c.solver != 1 && c.solver != 2) return EVH_ERR_SOLVER;
-    if (c.limiter < 1 || c.limiter > 3) return EVH_ERR_LIMITER;
-    if (c.solver != c0.solver || c.limiter != c0.limiter || c.precision != c0.precision ||
-        c.secondorder != c0.secondorder || c.dualenergy != c0.dualenergy || c.boundtype < 1 || c.boundtype > 3)
-      return EVH_ERR_CONFIG;  // one kernel variant for the whole ensemble
-  }
   evh_batch *b = new (std::nothrow) evh_batch();
   if (!b) return EVH_ERR_HIP;
   b->nbox = nbox;
@@ -720,6 +739,7 @@ int evh_batch_last_error(evh_batch *b, int *code, int *box, int cell[3], char *m
   }
   return EVH_OK;
 }
+
 
 }  // extern "C"
 #endif

@@ -1,7 +1,8 @@
 // fused.hip -- the whole 3-D hydro step of one grid in one HBM pass.
 //
 // k_fused runs the tile march of fused_core.h over one grid: one block per
-// (16 x TY tile, z chunk), with the tiles of each XCD contiguous.  k_cfl
+// (16 x TY tile, z chunk), with the tiles of each XCD contiguous; on 2-D
+// grids (nz = 1) one block per tile of the one plane.  k_cfl
 // gives the first step's CFL limit.  The file is compiled three times:
 // EVH_FUSED_PREC=64 (FP64, fast arithmetic), EVH_FUSED_PREC=32 (FP32-mixed)
 // and EVH_FUSED_EXACT=1 (FP64 with the reference's arithmetic, compiled
@@ -33,14 +34,16 @@ using core::Geo;
 
 // WPE: waves per SIMD the register allocation must leave room for
 // (512 / WPE VGPRs per lane); the tuning knob next to the tile height TY.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
+// D2: the 2-D form (nz = 1), the march without its z stage.
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE, bool D2>
 __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     K_FUSED(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
             S *__restrict__ tout) {
   __shared__ core::Smem<T, TY> sm;
   const int lin = xcd_tile(blockIdx.x, G.ntiles);
   if (lin < 0) return;  // padding block (whole block exits before any barrier)
-  core::march<T, S, SOLVER, LIMITER, TY, kExact>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm, tout);
+  core::march<T, S, SOLVER, LIMITER, TY, kExact, D2>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm,
+                                                     tout);
 }
 
 // primitive + CFL min over q without materialising w (first step, after
@@ -75,19 +78,24 @@ Geo make_geo(const evh_ctx *h, int ty) {
   return g;
 }
 
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE, bool D2 = false>
 void launch_ty(evh_ctx *h, double dt) {
   const Geo g = make_geo(h, TY);
   const int blocks = xcd_blocks(g.ntiles);
-  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
+  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE, D2>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
                      h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
                      (S *)h->tbuf);
 }
 
-// tuning variants: tile height 8 or 12, waves per SIMD (FP64: 2 or 3, FP32: 3 or 4)
+// tuning variants: tile height 8 or 12, waves per SIMD (FP64: 2 or 3, FP32: 3
+// or 4); the 2-D form in one shape (16 x 12 tiles, the higher budget)
 template <typename T, typename S, int SOLVER, int LIMITER>
 void launch_fused_t(evh_ctx *h, double dt) {
   constexpr int LO = sizeof(T) == 8 ? 2 : 3, HI = LO + 1;
+  if (h->grid.nz == 1) {
+    launch_ty<T, S, SOLVER, LIMITER, 12, HI, true>(h, dt);
+    return;
+  }
   const bool hi = h->wpe_hi;
   if (h->tile_ty == 8) {
     if (hi) launch_ty<T, S, SOLVER, LIMITER, 8, HI>(h, dt);
@@ -114,7 +122,8 @@ hipError_t launch_fused_prec(evh_ctx *h, double dt) {
 }  // namespace
 
 #if EVH_FUSED_PREC == 64 && !EVH_FUSED_EXACT
-bool fused_supported(const evh_config &c) { return c.nx >= 2 && c.ny >= 2 && c.nz >= 2; }
+// 3-D grids, and 2-D grids (nz = 1) in the march's 2-D form
+bool fused_supported(const evh_config &c) { return c.nx >= 2 && c.ny >= 2 && c.nz >= 1; }
 
 hipError_t launch_cfl(evh_ctx *h) {
   const Geo g = make_geo(h, 12);

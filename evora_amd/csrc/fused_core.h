@@ -27,7 +27,12 @@
 //       They sit where the interior waves have the least to do.
 // Template parameters: T compute type, S storage type of q (double/double
 // for the FP64 form, float/float for the FP32-mixed form), SOLVER, LIMITER,
-// TY the tile height (4, 8 or 12).
+// TY the tile height (4, 8 or 12), D2 the 2-D form.
+//
+// The 2-D form (nz = 1, muscl.f90:70-88 and the x/y sweeps of :171-274) is
+// the same march without its z stage: one plane, no z slopes, z faces or
+// carried partial update; evo = dt2 (a1 dxinv + a2 dyinv), and plane 0 is
+// finished in S4 as (q + dtx*(Fx)) + dty*(Fy).
 #pragma once
 
 #include "evh_internal.h"
@@ -156,9 +161,14 @@ struct Phys<false> {
 // The reference's HLL and HLLC (riemann.f90:47-153) with its expressions
 // and operation order (physics.h), written branch-free like the fast forms:
 // the side is selected first and one flux / state / star state is formed.
-// The entropy components take p / rho**g1 from the cube-root power
-// (fastmath.h); they are the only ones a power enters, and the device and
-// host powers differ in the last bits in any case.
+// Every quotient is the IEEE one, formed as fm::div_cr from the refined
+// reciprocal of its divisor (several quotients by one density share it),
+// and every square root as fm::sqrt_pos: the sequences the compiler emits
+// for IEEE division and square root, without their range scaling, so the
+// results are the IEEE ones for the normal operands of the hydro step.  The
+// entropy components take p / rho**g1 from the fast power (fastmath.h); they
+// are the only ones a power enters, and the device and host powers differ
+// in the last bits in any case.
 namespace exact_bf {
 template <int DIR>
 __device__ __forceinline__ void flux(const KParams &P, const double w[5], double sq, double f[6]) {
@@ -170,20 +180,25 @@ __device__ __forceinline__ void flux(const KParams &P, const double w[5], double
   f[4] = ud * (0.5 * w[0] * u2 + P.gg1 * w[4]);
   f[5] = ud * sq;
 }
+// state (riemann.f90:207-239); p/g1 from the correctly rounded 1/g1 (g1inv)
 __device__ __forceinline__ void state(const KParams &P, const double w[5], double sq, double q[6]) {
   q[0] = w[0];
   q[1] = w[0] * w[1];
   q[2] = w[0] * w[2];
   q[3] = w[0] * w[3];
   const double u2 = (w[1] * w[1]) + (w[2] * w[2]) + (w[3] * w[3]);
-  q[4] = 0.5 * w[0] * u2 + w[4] / P.g1;
+  q[4] = 0.5 * w[0] * u2 + fm::div_cr(w[4], P.g1, P.g1inv);
   q[5] = sq;
+}
+// sqrt(g p / rho) with y the refined reciprocal of rho
+__device__ __forceinline__ double sound(const KParams &P, double rho, double p, double y) {
+  return fm::sqrt_pos(fm::div_cr(P.g * p, rho, y));
 }
 template <int DIR>
 __device__ __forceinline__ int hll(const KParams &P, const double wl[5], const double wr[5], double f[6]) {
   int code = (wl[4] <= 0.0 || wr[4] <= 0.0) ? kErrRiemannPressure : kOk;
-  const double al = sqrt(P.g * wl[4] / wl[0]);
-  const double ar = sqrt(P.g * wr[4] / wr[0]);
+  const double al = sound(P, wl[0], wl[4], fm::rcp(wl[0]));
+  const double ar = sound(P, wr[0], wr[4], fm::rcp(wr[0]));
   const double sl = dmin(wl[DIR] - al, wr[DIR] - ar);
   const double sr = dmax(wl[DIR] + al, wr[DIR] + ar);
   const bool left = 0.0 < sl, right = 0.0 > sr, mid = sl <= 0.0 && 0.0 <= sr;
@@ -195,9 +210,11 @@ __device__ __forceinline__ int hll(const KParams &P, const double wl[5], const d
   flux<DIR>(P, wr, sqr, fr);
   state(P, wl, sql, ql);
   state(P, wr, sqr, qr);
+  const double den = sr - sl;
+  const double yd = fm::rcp(den);
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    const double fm_ = (sr * fl[i] - sl * fr[i] + sl * sr * (qr[i] - ql[i])) / (sr - sl);
+    const double fm_ = fm::div_cr(sr * fl[i] - sl * fr[i] + sl * sr * (qr[i] - ql[i]), den, yd);
     f[i] = left ? fl[i] : (right ? fr[i] : fm_);
   }
   return code;
@@ -205,13 +222,15 @@ __device__ __forceinline__ int hll(const KParams &P, const double wl[5], const d
 template <int DIR>
 __device__ __forceinline__ int hllc(const KParams &P, const double wl[5], const double wr[5], double f[6]) {
   int code = (wl[4] <= 0.0 || wr[4] <= 0.0) ? kErrRiemannPressure : kOk;
-  const double al = sqrt(P.g * wl[4] / wl[0]);
-  const double ar = sqrt(P.g * wr[4] / wr[0]);
+  const double yl = fm::rcp(wl[0]), yr = fm::rcp(wr[0]);
+  const double al = sound(P, wl[0], wl[4], yl);
+  const double ar = sound(P, wr[0], wr[4], yr);
   const double sl = dmin(wl[DIR] - al, wr[DIR] - ar);
   const double sr = dmax(wl[DIR] + al, wr[DIR] + ar);
   const double facl = wl[0] * (sl - wl[DIR]);
   const double facr = wr[0] * (sr - wr[DIR]);
-  const double sm = (wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR]) / (facl - facr);
+  const double fd = facl - facr;
+  const double sm = fm::div_cr(wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR], fd, fm::rcp(fd));
   const int br = (0.0 < sl) ? 0
                : (0.0 > sr) ? 1
                : (sl < 0.0 && 0.0 < sm) ? 2
@@ -221,6 +240,7 @@ __device__ __forceinline__ int hllc(const KParams &P, const double wl[5], const 
   double w[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) w[i] = right ? wr[i] : wl[i];
+  const double y = right ? yr : yl;
   const double sq = w[4] * fast::rpow_g1(P, w[0]);
   flux<DIR>(P, w, sq, f);
   // star state (statex, riemann.f90:241-288) and f + s (qx - q)
@@ -228,12 +248,14 @@ __device__ __forceinline__ int hllc(const KParams &P, const double wl[5], const 
   const double ud = w[DIR];
   double q[6], qx[6];
   state(P, w, sq, q);
-  qx[0] = w[0] * (s - ud) / (s - sm);
+  const double ssm = s - sm;
+  qx[0] = fm::div_cr(w[0] * (s - ud), ssm, fm::rcp(ssm));
 #pragma unroll
   for (int i = 1; i <= 3; i++) qx[i] = ((i == DIR) ? sm : w[i]) * qx[0];
-  const double energy = 0.5 * w[0] * ((w[1] * w[1]) + (w[2] * w[2]) + (w[3] * w[3])) + w[4] / P.g1;
-  qx[4] = qx[0] * (energy / w[0] + (sm - ud) * (sm + (w[4] / (w[0] * (s - ud)))));
-  qx[5] = qx[0] * (sq / w[0]);
+  const double energy = q[4];  // 0.5 rho u2 + p/g1, formed as in state
+  const double rsu = w[0] * (s - ud);
+  qx[4] = qx[0] * (fm::div_cr(energy, w[0], y) + (sm - ud) * (sm + fm::div_cr(w[4], rsu, fm::rcp(rsu))));
+  qx[5] = qx[0] * fm::div_cr(sq, w[0], y);
   const bool star = br >= 2;
 #pragma unroll
   for (int i = 0; i < 6; i++) f[i] = star ? f[i] + s * (qx[i] - q[i]) : f[i];
@@ -243,51 +265,39 @@ __device__ __forceinline__ int hllc(const KParams &P, const double wl[5], const 
 
 template <>
 struct Phys<true> {
-  // q2w + pressure (comp.f90:26-85) with the reference's IEEE arithmetic;
-  // the dual-energy branch takes rho**g1 from the cube-root power
+  // q2w + pressure (comp.f90:26-85): the fast form's FP64 primitive already
+  // forms every value as the reference does (IEEE quotients by rho from one
+  // reciprocal), and returns the correctly rounded 1/rho; the dual-energy
+  // branches take their powers from the fast form
   __device__ static __forceinline__ int q2w(const KParams &P, double q[6], double w[5], double &rinv) {
-    rinv = 0.0;  // the exact predictor divides by rho itself
-    const double densinv = 1.0 / q[0];
-    w[0] = q[0];
-    w[1] = q[1] * densinv;
-    w[2] = q[2] * densinv;
-    w[3] = q[3] * densinv;
-    const double m2 = (q[1] * q[1]) + (q[2] * q[2]) + (q[3] * q[3]);
-    const double kin = 0.5 * m2 / q[0];
-    double pr;
-    if (P.dualenergy) {
-      if (kin / q[4] > P.de_thresh1) {
-        pr = q[5] * fast::pow_g1(P, q[0]);
-        q[4] = kin + pr * P.g1inv;
-      } else {
-        pr = P.g1 * (q[4] - kin);
-        q[5] = pr * fast::rpow_g1(P, q[0]);
-      }
-    } else {
-      pr = P.g1 * (q[4] - kin);
-    }
-    w[4] = pr;
-    return (pr <= 0.0) ? kErrNegPressure : kOk;
+    return fast::q2w(P, q, w, rinv);
   }
   template <int LIMITER>
   __device__ static __forceinline__ double slope3(double wl, double wc, double wr) {
     return evh::slope3<LIMITER>(wl, wc, wr);
   }
+  // muscl.f90:100-104: s(5)/rho is the IEEE quotient from rinv
   template <int D>
-  __device__ static __forceinline__ void predictor_dir(const KParams &P, const double w[5], double, const double s[5],
-                                                       double a[5]) {
-    evh::predictor_dir<D>(P, w, s, a);
+  __device__ static __forceinline__ void predictor_dir(const KParams &P, const double w[5], double rinv,
+                                                       const double s[5], double a[5]) {
+    fast::predictor_dir<D>(P, w, rinv, s, a);
   }
   template <int SOLVER, int DIR>
   __device__ static __forceinline__ int riemann(const KParams &P, const double wl[5], const double wr[5], double f[6]) {
     if (SOLVER == 1) return exact_bf::hll<DIR>(P, wl, wr, f);
     return exact_bf::hllc<DIR>(P, wl, wr, f);
   }
-  __device__ static __forceinline__ double cfl_cell(const KParams &P, const double w[5], double) {
-    return evh::cfl_cell(P, w);
+  // timestep.f90:39-44 for one cell
+  __device__ static __forceinline__ double cfl_cell(const KParams &P, const double w[5], double rinv) {
+    const double a = exact_bf::sound(P, w[0], w[4], rinv);
+    const double sx = fabs(w[1]) + a, sy = fabs(w[2]) + a, sz = fabs(w[3]) + a;
+    const double vx = fm::div_cr(P.dx, sx, fm::rcp(sx));
+    const double vy = fm::div_cr(P.dy, sy, fm::rcp(sy));
+    const double vz = fm::div_cr(P.dz, sz, fm::rcp(sz));
+    return dmin(dmin(vx, vy), vz);
   }
-  __device__ static __forceinline__ double temp(const KParams &P, const double w[5], double) {
-    return w[4] / w[0] * P.temp0;
+  __device__ static __forceinline__ double temp(const KParams &P, const double w[5], double rinv) {
+    return fm::div_cr(w[4], w[0], rinv) * P.temp0;
   }
 };
 
@@ -364,7 +374,7 @@ struct Smem {
 // the tile's new state; errors carry the box index above bit 40 of the cell.
 // With tout (single box only), the temperature of each new cell is stored
 // there too, for the conduction pass that follows the sweep.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT, bool D2 = false>
 __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__restrict__ qin_all,
                                       S *__restrict__ qout_all, long long qoff, int box, double dt, Status *st,
                                       unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm,
@@ -443,29 +453,37 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   S pre[6], pre2[6];
   const long long own_xy = col_offset(P, G, gx, gy);
   const long long r2_xy = col_offset(P, G, x0 + r2x, y0 + r2y);
-  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
-  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
-  fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
   // the ring wave's second-ring cell: its w of the current plane (rw2) is
   // formed one plane ahead, in the ring wave's idle time of S2
   T rw2[5];
-  {
+#pragma unroll
+  for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
+  if constexpr (D2) {
+    // the one plane: its q (the base of the update), w and second ring
+    T rr2;
+    load_cell<EXACT, S, T>(P, G, qin, gx, gy, 0, qc, wc, rc);
+    fetch_at(G, qin, r2_xy, pre2);
+    prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
+  } else {
+    load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
+    load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
+    fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
     T rr2;
     fetch_at(G, qin, r2_xy + plane_offset(P, G, z0 - 1), pre2);
     prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
     fetch_at(G, qin, r2_xy + plane_offset(P, G, z0), pre2);
   }
-#pragma unroll
-  for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
 
-  for (int k = z0 - 1; k <= z1; k++) {
-    const bool out_plane = k >= z0 && k < z1;
+  for (int k = D2 ? 0 : z0 - 1; k <= (D2 ? 0 : z1); k++) {
+    const bool out_plane = D2 || (k >= z0 && k < z1);
     // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS.
     // Every lane forms and fetches (the ring wave's corner and idle lanes on
     // clamped coordinates), so that no branch makes a load wait for the
     // fetches issued after it.
-    prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
-    fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
+    if constexpr (!D2) {
+      prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
+      fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
+    }
     if (ring && r2x > -3) store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw2);
     if (owns_col) store_w<T, EA>(W, me, wc);
     __syncthreads();
@@ -478,14 +496,18 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     for (int m = 0; m < 5; m++) {
       sx[m] = Phys<EXACT>::template slope3<LIMITER>(W[m][me - 1], wc[m], W[m][me + 1]);
       sy[m] = Phys<EXACT>::template slope3<LIMITER>(W[m][me - EW], wc[m], W[m][me + EW]);
-      sz[m] = Phys<EXACT>::template slope3<LIMITER>(wm[m], wc[m], wp[m]);
+      sz[m] = D2 ? T(0) : Phys<EXACT>::template slope3<LIMITER>(wm[m], wc[m], wp[m]);
     }
     if (P.secondorder) {
       T a1[5], a2[5], a3[5];
       Phys<EXACT>::template predictor_dir<0>(P, wc, rc, sx, a1);
       Phys<EXACT>::template predictor_dir<1>(P, wc, rc, sy, a2);
-      Phys<EXACT>::template predictor_dir<2>(P, wc, rc, sz, a3);
-      fast::evo5(P, dt2, a1, a2, a3, ev);  // muscl.f90:106-107
+      if constexpr (D2) {
+        fast::evo5_2d(P, dt2, a1, a2, ev);  // muscl.f90:79-85
+      } else {
+        Phys<EXACT>::template predictor_dir<2>(P, wc, rc, sz, a3);
+        fast::evo5(P, dt2, a1, a2, a3, ev);  // muscl.f90:106-107
+      }
     } else {
 #pragma unroll
       for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }
@@ -517,7 +539,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     // z face k-1/2 (interior lanes) and the end of plane k-1; the
     // scheduling barrier keeps this solve apart from the x/y solves
     __builtin_amdgcn_sched_barrier(0);
-    if (!ring) {
+    if (!D2 && !ring) {
       T zl[5], zr[5];
 #pragma unroll
       for (int m = 0; m < 5; m++) {
@@ -545,7 +567,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       for (int m = 0; m < 5; m++) CR[12 + m][tid] = zr[m];
     }
     }  // owns_col
-    if (ring) {  // second-ring w of plane k+1, stored in S1 of plane k+1
+    if (!D2 && ring) {  // second-ring w of plane k+1, stored in S1 of plane k+1
       T rr2;
       prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
       fetch_at(G, qin, r2_xy + plane_offset(P, G, k + 2), pre2);
@@ -614,8 +636,19 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       T part[6];
 #pragma unroll
       for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
+      if constexpr (D2) {  // the plane is finished: no z faces
+        if (inside) {
 #pragma unroll
-      for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+          for (int m = 0; m < 6; m++) o[m] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+          oc = (long long)gx + (long long)G.nx * (long long)gy;
+#pragma unroll
+          for (int m = 0; m < 6; m++) qout[m * G.n + oc] = S(o[m]);
+          pend = true;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+      }
     }
     // the end of plane k-1: primitive of the new state for the CFL minimum
     // of the next step, its first negative pressure, its temperature
@@ -624,19 +657,21 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
 #pragma unroll
       for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
       const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
-      if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
+      if (ep && !nerr) { nerr = ep; nerr_k = D2 ? 0 : k - 1; }
       if constexpr (kSpeedKey) vmin = dmin(vmin, -fast::cfl_speed(P, wn, rn));
       else vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
       if (tout) tout[oc] = S(Phys<EXACT>::temp(P, wn, rn));
       pend = false;
     }
     // rotate the column window
+    if constexpr (!D2) {
 #pragma unroll
-    for (int m = 0; m < 5; m++) { wm[m] = wc[m]; wc[m] = wp[m]; }
-    rm = rc;
-    rc = rp;
+      for (int m = 0; m < 5; m++) { wm[m] = wc[m]; wc[m] = wp[m]; }
+      rm = rc;
+      rc = rp;
 #pragma unroll
-    for (int m = 0; m < 6; m++) qc[m] = qn[m];
+      for (int m = 0; m < 6; m++) qc[m] = qn[m];
+    }
   }
 
   if (err || nerr) {  // the lane's own column (clamped into the box) locates the error

@@ -3,7 +3,9 @@
 //
 // Operand order follows the reference expression by expression, so that a
 // translation unit built with -ffp-contract=off reproduces the reference's
-// IEEE double results (the staged kernels are checked bit for bit against
+// IEEE double results (the primitive and CFL functions, which the fast
+// translation units call for the first step's limit, turn contraction off
+// themselves) (the staged kernels are checked bit for bit against
 // the CPU restatement).  Fused kernels include the same code; where they
 // trade exactness for speed it is a named alternative below, not an edit.
 #pragma once
@@ -75,6 +77,7 @@ __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
 // pressure(q, press); q may be rewritten when dualenergy is on.
 template <typename T>
 __device__ __forceinline__ int pressure(const KParams &P, T q[6], T &press) {
+#pragma clang fp contract(off)
   const T m2 = (q[1] * q[1]) + (q[2] * q[2]) + (q[3] * q[3]);
   const T kin = T(0.5) * m2 / q[0];
   const T ratio = kin / q[4];
@@ -97,6 +100,7 @@ __device__ __forceinline__ int pressure(const KParams &P, T q[6], T &press) {
 // comp.f90:26-43  q2w
 template <typename T>
 __device__ __forceinline__ int q2w(const KParams &P, T q[6], T w[5]) {
+#pragma clang fp contract(off)
   const T densinv = T(1) / q[0];
   w[0] = q[0];
   w[1] = q[1] * densinv;
@@ -108,6 +112,7 @@ __device__ __forceinline__ int q2w(const KParams &P, T q[6], T w[5]) {
 // timestep.f90:39-44: min over directions of d/(|u|+a) for one cell
 template <typename T>
 __device__ __forceinline__ double cfl_cell(const KParams &P, const T w[5]) {
+#pragma clang fp contract(off)
   const double a = sqrt(P.g * double(w[4]) / double(w[0]));
   const double vx = P.dx / (fabs(double(w[1])) + a);
   const double vy = P.dy / (fabs(double(w[2])) + a);

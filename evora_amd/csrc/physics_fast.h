@@ -156,6 +156,19 @@ __device__ __forceinline__ void evo5(const KParams &P, double dt2, const double 
 #pragma unroll
   for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * P.dxinv + a2[m] * P.dyinv + a3[m] * P.dzinv);
 }
+// the 2-D form (muscl.f90:79-85): dt2 (a1 dxinv + a2 dyinv)
+__device__ __forceinline__ void evo5_2d(const KParams &P, double dt2, const double a1[5], const double a2[5],
+                                        double ev[5]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int m = 0; m < 5; m++) ev[m] = dt2 * (a1[m] * P.dxinv + a2[m] * P.dyinv);
+}
+__device__ __forceinline__ void evo5_2d(const KParams &P, float dt2, const float a1[5], const float a2[5],
+                                        float ev[5]) {
+  const float ex = dt2 * float(P.dxinv), ey = dt2 * float(P.dyinv);
+#pragma unroll
+  for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey;
+}
 __device__ __forceinline__ void evo5(const KParams &P, float dt2, const float a1[5], const float a2[5],
                                      const float a3[5], float ev[5]) {
   const float ex = dt2 * float(P.dxinv), ey = dt2 * float(P.dyinv), ez = dt2 * float(P.dzinv);

@@ -36,6 +36,7 @@ module evora_hip
      real(c_double) :: grav, fb, h2, kfx2, kfy2, kfz2
      integer(c_int) :: nyqx, nyqy, nyqz
      real(c_double) :: dx2inv, dy2inv, dz2inv
+     integer(c_int) :: hydro_off
   end type evh_config
 
   interface

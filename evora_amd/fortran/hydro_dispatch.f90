@@ -32,8 +32,10 @@
 ! (tend - t <= dt, the test of timestep.f90:163).  w, evo and s are never
 ! filled on the host; pure hydro never reads them there.  Thermal
 ! conduction (thermalstep, thermalconduction with simplekappa, without dual
-! energy) and self-gravity on 3-D grids (gravpotential, gravintegration; the
-! potential stays on the device) run on the device too.  Physics that
+! energy; on 1-, 2- and 3-D grids, also with hydro = .false., when main.f90
+! calls neither hydrostep nor sweep) and self-gravity on 3-D grids
+! (gravpotential, gravintegration; the potential stays on the device) run
+! on the device too.  Physics that
 ! update q on the host between steps (chemistry, floors), supercomoving
 ! runs (whose host-side time and snapshot rules this dispatch does not
 ! mirror), conduction with the full kappa or with dual energy, and
@@ -132,7 +134,7 @@ contains
     use global
     type(evh_config) :: cfg
     if (chemistry .or. supercomoving .or. tempfloor .or. jeansfloor &
-         .or. cooldt .or. noncie .or. parallel .or. .not. hydro) then
+         .or. cooldt .or. noncie .or. parallel .or. (.not. hydro .and. .not. conduction)) then
        print *, 'ERROR: EVORA_BACKEND=hip runs hydro (with thermal conduction and gravity) on one rank only'
        call cleanstop
     endif
@@ -166,11 +168,21 @@ contains
     cfg%kfx2 = kfx2 ; cfg%kfy2 = kfy2 ; cfg%kfz2 = kfz2
     cfg%nyqx = nyqx ; cfg%nyqy = nyqy ; cfg%nyqz = nyqz
     cfg%dx2inv = dx2inv ; cfg%dy2inv = dy2inv ; cfg%dz2inv = dz2inv
+    cfg%hydro_off = merge(0, 1, hydro)
     call hip_check(evh_create(cfg, ctx))
-    ! 3-D grids start in the fused form; the exact form on request (FP64)
-    if (exact .and. precision .eq. EVH_FP64 .and. .not. onedimension .and. .not. twodimension) &
+    ! 2-D and 3-D grids start in the fused form; the exact form on request (FP64)
+    if (exact .and. precision .eq. EVH_FP64 .and. .not. onedimension) &
          call hip_check(evh_set_mode(ctx, EVH_MODE_FUSED_EXACT))
   end subroutine hip_create
+
+  ! one more time step begins (the step's first time limit was taken):
+  ! EVORA_MAXSTEPS and the last-step test of timestep.f90:163
+  subroutine count_step()
+    use global
+    nsteps_seen = nsteps_seen + 1
+    if (maxsteps .ge. 0 .and. nsteps_seen .gt. maxsteps) tend = t
+    last_step = (tend - t .le. dt)
+  end subroutine count_step
 
   ! main.f90:181-195 after this step (t already advanced by updatetime):
   ! download q when the driver is about to read it; evolution takes its
@@ -237,9 +249,7 @@ subroutine hydrostep(w)
   else
      call hip_check(evh_hydrostep(ctx, dt))
   endif
-  nsteps_seen = nsteps_seen + 1
-  if (maxsteps .ge. 0 .and. nsteps_seen .gt. maxsteps) tend = t
-  last_step = (tend - t .le. dt)
+  call count_step()
 end subroutine hydrostep
 
 subroutine compevo(w, evo, s)
@@ -359,9 +369,11 @@ subroutine thermalstep(q, w)
   call decide_backend()
   if (backend .eq. 0) then
      call evcpu_thermalstep(q, w)
-     return
+  else
+     call hip_check(evh_thermalstep(ctx, dt))
   endif
-  call hip_check(evh_thermalstep(ctx, dt))
+  ! without hydro (main.f90:119) this is the step's first limit
+  if (.not. hydro) call count_step()
   last_step = (tend - t .le. dt)  ! dt may have shrunk: the test of timestep.f90:163 again
 end subroutine thermalstep
 

@@ -40,7 +40,7 @@ EXPORTS = (
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
     "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_conduction_stats", "evh_stage_stats",
     "evh_gravpotential", "evh_gravintegration", "evh_download_phi",
-    "evh_device_bytes",
+    "evh_device_bytes", "evh_math_eval",
     "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
     "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
     "evh_batch_get_tuning", "evh_batch_autotune", "evh_batch_last_error",
@@ -67,6 +67,7 @@ class EvhConfig(C.Structure):
         ("kfx2", C.c_double), ("kfy2", C.c_double), ("kfz2", C.c_double),
         ("nyqx", C.c_int), ("nyqy", C.c_int), ("nyqz", C.c_int),
         ("dx2inv", C.c_double), ("dy2inv", C.c_double), ("dz2inv", C.c_double),
+        ("hydro_off", C.c_int),
     ]
 
 
@@ -120,6 +121,8 @@ def load_library():
             fn.restype = C.c_int
         L.evh_device_bytes.argtypes = [P]
         L.evh_device_bytes.restype = C.c_longlong
+        L.evh_math_eval.argtypes = [C.c_int, C.c_longlong, D, D, D]
+        L.evh_math_eval.restype = C.c_int
         _lib = L
     return _lib
 
@@ -134,6 +137,20 @@ def make_config(d: Derived, precision: int = FP64, mach_fp64: float = 5.0) -> Ev
         else:
             setattr(c, name, getattr(d, name))
     return c
+
+
+MATH_POW, MATH_POW_G1, MATH_RPOW_G1, MATH_DIV, MATH_RCP, MATH_SQRT, MATH_POWF = range(7)
+
+
+def math_eval(fn: int, x, y=None) -> np.ndarray:
+    """The fast forms' device arithmetic f(x, y) elementwise (evh_math_eval)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.empty_like(x)
+    code = load_library().evh_math_eval(fn, x.size, _dptr(x), _dptr(y), _dptr(out))
+    if code:
+        raise EvoraError(code, "evh_math_eval failed")
+    return out
 
 
 def _dptr(a: np.ndarray):

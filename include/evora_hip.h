@@ -56,7 +56,7 @@ typedef struct evh_config {
   double volume;
   double mach_fp64; /* FP32_MIXED: pressure from E in FP64 where Mach > mach_fp64 (e.g. 5) */
   /* thermal conduction (parameter.f90:77-78, units.f90:50,63): only
-   * simplekappa, without dual energy, on 3-D grids */
+   * simplekappa, without dual energy; 1-D and 2-D grids in EVH_MODE_STAGED */
   int conduction, simplekappa;
   double c_cond, temp0, kappa0;
   /* self-gravity (parameter.f90:235-241,254-263): 3-D, one rank; the
@@ -65,6 +65,9 @@ typedef struct evh_config {
   double grav, fb, h2, kfx2, kfy2, kfz2;
   int nyqx, nyqy, nyqz;
   double dx2inv, dy2inv, dz2inv;
+  /* 1: the namelist's hydro = .false. (parameter.f90:44): evh_step then skips
+   * hydrostep and sweep, as main.f90:119,155 do (conduction-only runs) */
+  int hydro_off;
 } evh_config;
 
 typedef struct evh_ctx evh_ctx;
@@ -155,6 +158,21 @@ int evh_stage_stats(evh_ctx *h, int stage, double *total_ms, long long *launches
 
 /* Bytes of device memory the context holds. */
 long long evh_device_bytes(evh_ctx *h);
+
+/* Diagnostics (no counterpart in the reference): out[i] = f(x[i], y[i]) for
+ * the device arithmetic of the fast forms, on host arrays of n values, so
+ * that it can be held against the host's IEEE operations and libm:
+ *   EVH_MATH_POW      x**y for x > 0 (exp(y ln x), the general power)
+ *   EVH_MATH_POW_G1   x**g1 with g1 = y (the cube-root form when g1 is 2/3)
+ *   EVH_MATH_RPOW_G1  x**(-g1) with g1 = y
+ *   EVH_MATH_DIV      x / y from the refined reciprocal of y (IEEE quotient)
+ *   EVH_MATH_RCP      1 / x, correctly rounded
+ *   EVH_MATH_SQRT     sqrt(x) for x > 0
+ *   EVH_MATH_POWF     x**y in single precision (hardware exp2 and log2)
+ * Allocates and frees device memory; synchronous. */
+enum { EVH_MATH_POW = 0, EVH_MATH_POW_G1 = 1, EVH_MATH_RPOW_G1 = 2, EVH_MATH_DIV = 3, EVH_MATH_RCP = 4,
+       EVH_MATH_SQRT = 5, EVH_MATH_POWF = 6 };
+int evh_math_eval(int fn, long long n, const double *x, const double *y, double *out);
 
 /* ---- batched form (no counterpart in the reference, whose equivalent is
  * one run per box): many independent 3-D boxes of uneven sizes, each its own

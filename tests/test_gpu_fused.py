@@ -160,3 +160,52 @@ def test_fused_smallest_grids(shape, bt):
         else:
             for c in range(6):
                 assert rel_linf(q[..., c], qo[..., c]) <= 1e-12, (mode, prec, c)
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("limiter", [1, 2, 3])
+@pytest.mark.parametrize("bt", [1, 2, 3])
+def test_fused_2d(solver, limiter, bt):
+    """The single-pass 2-D form (nz = 1, muscl.f90:70-88 and the x/y sweeps):
+    one step of the fast form to 1e-12 per component, the exact form equal to
+    the oracle bit for bit in q1-q5 over 3 steps, on a grid that is not a
+    multiple of the 16 x 12 tile."""
+    p = HydroParams(nx=37, ny=29, nz=1, solver=solver, limiter=limiter, boundtype=bt, cfl=0.4, atomic=True)
+    d = p.derive()
+    q0 = ics.random_smooth(p, seed=300 + 100 * solver + 10 * limiter + bt)
+    q1, q1_o, dt, dt_o, _ = one_step(p, q0)
+    assert dt == dt_o
+    for c in range(6):
+        assert rel_linf(q1[..., c], q1_o[..., c]) <= ONE_STEP_TOL, c
+    qo = q0.copy()
+    to, dto, _ = O.Oracle(d).run(qo, 3)
+    with hydro.Hydro(d, mode=hydro.MODE_FUSED_EXACT) as h:
+        h.upload_q(q0)
+        t, dt, n = h.run(3)
+        q = h.download_q()
+    assert n == 3 and (t, dt) == (to, dto)
+    for c in range(5):
+        assert np.array_equal(q[..., c], qo[..., c]), c
+    assert rel_linf(q[..., 5], qo[..., 5]) <= 1e-13
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.FP32_MIXED])
+def test_fused_2d_run(precision):
+    """20 steps of the 2-D form on a 130 x 70 grid (partial tiles in x and y),
+    fast FP64 to relative L1 1e-10, FP32-mixed to 1e-5 in rho and p."""
+    p = HydroParams(nx=130, ny=70, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.4, atomic=True)
+    d = p.derive()
+    q0 = ics.random_smooth(p, seed=77, amp=0.1)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 20)
+    with hydro.Hydro(d, precision=precision) as h:
+        h.upload_q(q0)
+        h.run(20)
+        q = h.download_q()
+    if precision == hydro.FP64:
+        for c in range(6):
+            assert rel_l1(q[..., c], qo[..., c]) <= RUN_L1_TOL, c
+    else:
+        kin = lambda a: 0.5 * (a[..., 1] ** 2 + a[..., 2] ** 2 + a[..., 3] ** 2) / a[..., 0]
+        assert rel_l1(q[..., 0], qo[..., 0]) <= 1e-5
+        assert rel_l1(q[..., 4] - kin(q), qo[..., 4] - kin(qo)) <= 1e-5
