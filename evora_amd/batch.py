@@ -37,6 +37,8 @@ def _lib():
             "evh_batch_get_tuning": [P, I, I, I, I],
             "evh_batch_autotune": [P, C.c_int, C.c_char_p, C.c_int],
             "evh_batch_last_error": [P, I, I, I, C.c_char_p, C.c_int],
+            "evh_batch_run_noalloc": [C.POINTER(EvhConfig), C.c_int, P, P, I, D, D, I, I, C.c_int, C.c_char_p,
+                                      C.c_int],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -133,3 +135,27 @@ class Batch:
         buf = C.create_string_buffer(1 << 16)
         self._check(self.L.evh_batch_autotune(self.h, trial_steps, buf, 1 << 16))
         return buf.value.decode()
+
+
+def run_noalloc(params, q0_ptr: int, q1_ptr: int, side, t, tend, steps, go, nsteps: int,
+                precision: int = FP64, mach_fp64: float = 5.0) -> int:
+    """evh_batch_run_noalloc: nsteps batched steps on the caller's device
+    buffers (addresses q0_ptr, q1_ptr: box i's SoA state at element offset
+    6 * sum of the earlier boxes' cells, in buffer side[i]), with nothing
+    allocated on the device.  side, t, steps, go are updated in place
+    (numpy int32 / float64 arrays); returns the status (0 or the first
+    box's error code) and raises on argument errors."""
+    L = _lib()
+    d = [p.derive() if isinstance(p, HydroParams) else p for p in params]
+    cfgs = (EvhConfig * len(d))(*[make_config(x, precision, mach_fp64) for x in d])
+    I = C.POINTER(C.c_int)
+    err = C.create_string_buffer(256)
+    for a, dt_ in ((side, np.int32), (t, np.float64), (tend, np.float64), (steps, np.int32), (go, np.int32)):
+        if a.dtype != dt_ or not a.flags.c_contiguous or a.size != len(d):
+            raise ValueError("side/steps/go: int32, t/tend: float64, one value per box")
+    code = L.evh_batch_run_noalloc(cfgs, len(d), C.c_void_p(q0_ptr), C.c_void_p(q1_ptr), side.ctypes.data_as(I),
+                                   _dptr(t), _dptr(tend), steps.ctypes.data_as(I), go.ctypes.data_as(I), nsteps,
+                                   err, 256)
+    if code in (8, 9, 10):
+        raise EvoraError(code, err.value.decode())
+    return code

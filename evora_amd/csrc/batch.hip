@@ -741,5 +741,164 @@ int evh_batch_last_error(evh_batch *b, int *code, int *box, int cell[3], char *m
 }
 
 
+// ---------------------------------------------------------------- allocation-free form
+// evh_batch_run_noalloc: the batched step for callers that cannot allocate
+// device memory.  Everything the kernels need beyond the caller's two state
+// buffers -- box descriptors, work queues, per-box time, dt, flags and status
+// blocks -- lives in `scratch`, a fixed device array of this file (one copy
+// per GPU, for the life of the process).  Every call rewrites all of it
+// before the first kernel reads it; calls that overlap on one GPU race on it.
+constexpr int kNoAllocMaxBoxes = 64;
+
+struct NoAllocScratch {
+  BoxDev boxes[kNoAllocMaxBoxes];
+  Queues qs;
+  double t[kNoAllocMaxBoxes], tend[kNoAllocMaxBoxes], dt[kNoAllocMaxBoxes], cfl[kNoAllocMaxBoxes];
+  int go[kNoAllocMaxBoxes], active[kNoAllocMaxBoxes], steps[kNoAllocMaxBoxes], err_step[kNoAllocMaxBoxes];
+  int side[kNoAllocMaxBoxes];
+  Status st[kNoAllocMaxBoxes];
+};
+constexpr size_t kScratchBytes = 48 * 1024;
+static_assert(sizeof(NoAllocScratch) <= kScratchBytes, "scratch too small for its layout");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *scratch_ptr[kMaxDevices] = {};
+
+// the device address of this GPU's copy of `scratch`, looked up once per
+// device (the host-side name is not a device address)
+hipError_t scratch_address(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return hipSuccess;
+}
+
+int na_fail(char *err, int errlen, int code, const char *msg) {
+  if (err && errlen > 0) {
+    std::strncpy(err, msg, (size_t)errlen - 1);
+    err[errlen - 1] = 0;
+  }
+  return code;
+}
+}  // namespace
+
+int evh_batch_run_noalloc(const evh_config *cfgs, int nbox, void *q0, void *q1, int *side, double *t,
+                          const double *tend, int *steps, int *go, int nsteps, char *err, int errlen) {
+  if (err && errlen > 0) err[0] = 0;
+  if (!cfgs || !q0 || !q1 || !side || !t || !tend || !steps || !go || nsteps < 0)
+    return na_fail(err, errlen, EVH_ERR_STATE, "null argument or negative nsteps");
+  if (nbox < 1 || nbox > kNoAllocMaxBoxes) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "nbox %d outside 1..%d, the boxes the allocation-free scratch (%zu bytes) holds",
+                  nbox, kNoAllocMaxBoxes, kScratchBytes);
+    return na_fail(err, errlen, EVH_ERR_CONFIG, msg);
+  }
+  const int v = check_configs(cfgs, nbox);
+  if (v) return na_fail(err, errlen, v, "unsupported box configuration");
+  for (int i = 0; i < nbox; i++)
+    if (side[i] != 0 && side[i] != 1) return na_fail(err, errlen, EVH_ERR_STATE, "side must be 0 or 1");
+  void *sp = nullptr;
+  hipError_t e = scratch_address(&sp);
+  if (e != hipSuccess) return na_fail(err, errlen, EVH_ERR_HIP, hipGetErrorString(e));
+  NoAllocScratch *dev = (NoAllocScratch *)sp;
+
+  // the whole scratch image, formed on the host stack (no heap either)
+  NoAllocScratch img;
+  std::memset(&img, 0, sizeof img);
+  KParams kps[kNoAllocMaxBoxes];
+  long long ncell[kNoAllocMaxBoxes], qoff[kNoAllocMaxBoxes];
+  long long off = 0;
+  const bool f32 = cfgs[0].precision == EVH_FP32_MIXED;
+  for (int i = 0; i < nbox; i++) {
+    kps[i] = make_kparams(cfgs[i]);
+    ncell[i] = (long long)cfgs[i].nx * cfgs[i].ny * cfgs[i].nz;
+    qoff[i] = off;
+    off += ncell[i];
+  }
+  // the launch shape evh_batch_create starts with
+  const int ty = 12, zc = 64, wpe_hi = 1, bpc = f32 ? 4 : 2;
+  describe_boxes(cfgs, kps, ncell, qoff, nbox, ty, zc, img.boxes, &img.qs);
+  for (int i = 0; i < nbox; i++) {
+    img.t[i] = t[i];
+    img.tend[i] = tend[i];
+    img.dt[i] = 0.0;
+    img.cfl[i] = cfgs[i].cfl;
+    img.go[i] = go[i] ? 1 : 0;
+    img.steps[i] = steps[i];
+    img.err_step[i] = -1;
+    img.side[i] = side[i];
+  }
+  BoxState bs;
+  bs.t = dev->t;
+  bs.tend = dev->tend;
+  bs.dt = dev->dt;
+  bs.cfl = dev->cfl;
+  bs.go = dev->go;
+  bs.active = dev->active;
+  bs.steps = dev->steps;
+  bs.err_step = dev->err_step;
+  bs.side = dev->side;
+  bs.st = dev->st;
+  int devid = 0, cu = 256;
+  if ((e = hipGetDevice(&devid)) == hipSuccess)
+    e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, devid);
+  if (e == hipSuccess) e = hipMemcpy(dev, &img, sizeof img, hipMemcpyHostToDevice);
+  // the first CFL limit of every running box from its current state
+  for (int i = 0; i < nbox && e == hipSuccess; i++) {
+    if (!img.go[i]) continue;
+    hipLaunchKernelGGL(k_b_reset_status, dim3(1), dim3(64), 0, 0, bs.st, i);
+    const long long n = ncell[i];
+    if (f32)
+      hipLaunchKernelGGL(k_batch_cfl<float>, dim3(gridsz(n)), dim3(256), 0, 0, kps[i], n,
+                         (const float *)(side[i] ? q1 : q0) + 6 * qoff[i], bs.st + i);
+    else
+      hipLaunchKernelGGL(k_batch_cfl<double>, dim3(gridsz(n)), dim3(256), 0, 0, kps[i], n,
+                         (const double *)(side[i] ? q1 : q0) + 6 * qoff[i], bs.st + i);
+    e = hipGetLastError();
+  }
+  const evh_config &c0 = cfgs[0];
+  for (int k = 0; k < nsteps && e == hipSuccess; k++) {
+    hipLaunchKernelGGL(k_batch_dt, dim3((nbox + 255) / 256), dim3(256), 0, 0, nbox, bs, &dev->qs);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    if (f32)
+      e = launch_batch_f32(c0.solver, c0.limiter, ty, wpe_hi != 0, dev->boxes, nbox, q0, q1, bs, &dev->qs,
+                           bpc * cu, 0);
+    else
+      e = launch_batch_f64(c0.solver, c0.limiter, ty, wpe_hi != 0, dev->boxes, nbox, q0, q1, bs, &dev->qs,
+                           bpc * cu, 0);
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(&img, dev, sizeof img, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return na_fail(err, errlen, EVH_ERR_HIP, hipGetErrorString(e));
+  int code = EVH_OK;
+  for (int i = 0; i < nbox; i++) {
+    t[i] = img.t[i];
+    steps[i] = img.steps[i];
+    go[i] = img.go[i];
+    side[i] = img.side[i];
+    const Status &st = img.st[i];
+    const int c = st.code ? st.code : (int)st.next_code;
+    if (c && !code) {
+      code = c;
+      char msg[96];
+      std::snprintf(msg, sizeof msg, "box %d stopped with status %d", i, c);
+      na_fail(err, errlen, c, msg);
+    }
+  }
+  return code;
+}
+
 }  // extern "C"
 #endif

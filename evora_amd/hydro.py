@@ -43,7 +43,7 @@ EXPORTS = (
     "evh_device_bytes", "evh_math_eval",
     "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
     "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
-    "evh_batch_get_tuning", "evh_batch_autotune", "evh_batch_last_error",
+    "evh_batch_get_tuning", "evh_batch_autotune", "evh_batch_last_error", "evh_batch_run_noalloc",
 )
 
 

@@ -203,6 +203,24 @@ int evh_batch_get_tuning(evh_batch *b, int *tile_ty, int *wpe_hi, int *zchunk, i
 int evh_batch_autotune(evh_batch *b, int trial_steps, char *log, int loglen);
 int evh_batch_last_error(evh_batch *b, int *code, int *box, int cell[3], char *msg, int msglen);
 
+/* Allocation-free batched run.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same steps as
+ * evh_batch_create + evh_batch_run(nsteps, eager) on the caller's own device
+ * buffers q0 and q1, each holding the boxes' SoA states (box i: 6 arrays of
+ * nx*ny*nz values, [6][nz][ny][nx], in the storage precision, at element
+ * offset 6 * the cells of boxes 0..i-1) in the buffer side[i] names.  Every
+ * running box (go[i] != 0) first takes its CFL limit from its current state,
+ * as after an upload.  side, t, steps and go are updated in place (tend is
+ * read); the return value is EVH_OK or the first stopped box's status, whose
+ * message goes to err (may be NULL).  Descriptors, work queues and per-box
+ * status live in a fixed 48 KiB device array of the library, one per GPU:
+ * at most 64 boxes (more is rejected with EVH_ERR_CONFIG); no memory is
+ * allocated or freed.  Calls that overlap on one GPU race on that array.
+ * Synchronous (the default stream). */
+int evh_batch_run_noalloc(const evh_config *cfgs, int nbox, void *q0, void *q1, int *side, double *t,
+                          const double *tend, int *steps, int *go, int nsteps, char *err, int errlen);
+
 #ifdef __cplusplus
 }
 #endif
