@@ -22,7 +22,8 @@ hydro step alone, and the conduction kernel, on a 512^3 hot blob), the step
 with self-gravity on (against the hydro step alone, with the potential and
 the force kernel timed, on a 512^3 Jeans-unstable box), and the reference's
 CPU hot path timed on the host's cores on a bounded sample.
-Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
+The single-pass 2-D form is timed on the 8192^2 shear layer of
+para/kevinhelmholtz (forms.two_d).  Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
 HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50 steps, the
 cosmological-like field 256^3 for 100 steps, the Jeans box with gravity and
 the hot blob with conduction at 256^3 for 50 steps, and the batched
@@ -229,12 +230,30 @@ def compare_forms(d, r64, r32):
     return out
 
 
-def accuracy_case(name, d, q0, steps, control=False):
+def run_fp64_rounded_each_step(d, q0, steps):
+    """FP64 steps with the state rounded to single precision before every
+    step (downloaded, rounded, uploaded): what the FP32 storage of the mixed
+    form alone does to a run, its arithmetic aside."""
+    from evora_amd import hydro
+
+    with hydro.Hydro(d, precision=hydro.FP64) as h:
+        q = q0.astype(np.float32).astype(np.float64)
+        t = 0.0
+        for _ in range(steps):
+            h.upload_q(q)
+            t, dt, go = h.step(t, 1e30)
+            q = h.download_q().astype(np.float32).astype(np.float64)
+    return q
+
+
+def accuracy_case(name, d, q0, steps, control=False, storage_control=False):
     """One configuration advanced `steps` steps in both forms; returns the
     FP32-mixed errors against FP64.  With `control`, FP64 is also run from
     the initial state rounded to single precision: its L1 distance from the
     FP64 run is what the rounding of the input alone grows to in `steps`
-    steps (large for unstable flows such as a shear layer)."""
+    steps (large for unstable flows such as a shear layer).  With
+    `storage_control`, FP64 is run with the state rounded to single precision
+    before every step: the error of the FP32 storage alone."""
     from evora_amd.hydro import EvoraError
 
     try:
@@ -247,6 +266,10 @@ def accuracy_case(name, d, q0, steps, control=False):
             e = L1Error(d)
             e.add(rc["q"], r64["q"])
             out["fp64_from_rounded_input"] = e.result()
+        if storage_control:
+            e = L1Error(d)
+            e.add(run_fp64_rounded_each_step(d, q0, steps), r64["q"])
+            out["fp64_rounded_each_step"] = e.result()
     except EvoraError as e:  # the scheme stopped, as the reference would
         log(f"accuracy {name}: stopped: {e}")
         return dict(stopped=str(e), steps=steps)
@@ -329,6 +352,31 @@ def batch_accuracy(params, q0, b32, nsteps):
     return out
 
 
+def run_2d(steps, warmup, fp64_only, n=8192):
+    """The single-pass 2-D form (nz = 1) on the shear layer of
+    para/kevinhelmholtz (ic/kevinhelmholtz.f90; gamma 1.4, HLLC, MC limiter,
+    periodic, CFL 0.5) at n x n: the fast FP64, exact FP64 and FP32-mixed
+    forms, with the kernel's HBM roofline at 96 / 48 B per cell."""
+    from evora_amd import ics
+    from evora_amd.params import HydroParams
+
+    p = HydroParams(nx=n, ny=n, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.5, atomic=False)
+    d = p.derive()
+    log(f"building {n}^2 Kelvin-Helmholtz initial condition")
+    q0 = ics.kh_reference(p)
+    out = {"workload": f"2-D Kelvin-Helmholtz {n}x{n}x1 (para/kevinhelmholtz), HLLC, MC, CFL 0.5, periodic",
+           "cells": n * n}
+    forms = [("fp64", "fp64", False), ("fp64_exact", "fp64", True)]
+    if not fp64_only:
+        forms.append(("fp32_mixed", "fp32_mixed", False))
+    for name, prec, exact in forms:
+        r = run_gpu(d, q0, steps, warmup, prec, exact=exact)
+        out[name] = dict(value=r["value"], ms_per_step=r["ms_per_step"], kernel_ms=r["kernel_ms"],
+                         hbm_frac=BYTES_PER_CELL[prec] * n * n / (r["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        log(f"2-D {name}: {r['value'] / 1e9:.2f} G cell-updates/s, kernel {r['kernel_ms']:.3f} ms")
+    return out
+
+
 def cpu_baseline(n, cores, nsteps=2):
     """The reference's own hot path (oracle/_ref/evref, compiled from the
     reference Fortran at -O3) on `cores` processes, each advancing one z-slab
@@ -382,14 +430,15 @@ def cpu_baseline(n, cores, nsteps=2):
         physical = psutil.cpu_count(logical=False)
     except Exception:  # noqa: BLE001 -- the count is informative only
         physical = None
+    usable = len(os.sched_getaffinity(0))
     return dict(value=n ** 3 * nsteps / slowest, unit="cell-updates/s", cores=cores, kind="reference",
                 sample=f"{nsteps} steps of the {n}^3 HLLC blast, one reference process per z-slab "
                        f"({cores} slabs, own ghost fill, no halo exchange), slowest slab {slowest:.2f} s",
-                host_logical_cpus=os.cpu_count(), host_physical_cores=physical,
-                cpus_usable=len(os.sched_getaffinity(0)),
+                host_logical_cpus=os.cpu_count(), host_physical_cores=physical, cpus_usable=usable,
                 compiler="AMD flang -O3 (gfortran is not in the image)",
-                note=f"{cores} processes: the host-CPU share of one GPU on this machine; the remaining "
-                     "cores of the host were left idle")
+                note=(f"{cores} processes on the {usable} CPUs the process may use (the host-CPU share of one GPU)"
+                      if cores >= usable else f"{cores} processes; {usable - cores} of the {usable} usable CPUs "
+                      "were left idle"))
 
 
 def copy_probe():
@@ -422,6 +471,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-mixed against FP64 error and drift runs")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2-D form")
     ap.add_argument("--no-conduction", action="store_true", help="skip the thermal-conduction form")
     ap.add_argument("--conduction-only", action="store_true",
                     help="run only the thermal-conduction form and print its JSON (for profiling)")
@@ -484,7 +534,7 @@ def main():
             accuracy["cosmo256"] = accuracy_case("cosmo", cp.derive(), ics.cosmo_field(cp), 100, control=True)
             jp = ics.jeans_params(256)
             accuracy["jeans256_gravity"] = accuracy_case("jeans gravity", jp.derive(), ics.jeans(jp), 50,
-                                                         control=True)
+                                                         control=True, storage_control=True)
             bp = ics.conduction_params(256, boundtype=2)
             accuracy["blob256_conduction"] = accuracy_case("blob conduction", bp.derive(), ics.conduction_blob(bp),
                                                            50, control=True)
@@ -494,6 +544,8 @@ def main():
             os.makedirs(os.path.dirname(os.path.abspath(args.autotune_log)), exist_ok=True)
         forms["batched"] = run_batch(args.batch_boxes, args.batch_steps, 2, log_path=args.autotune_log,
                                      accuracy=acc)
+    if not args.no_2d:
+        forms["two_d"] = run_2d(args.steps, args.warmup, args.fp64_only)
     if not args.no_conduction:
         forms["conduction"] = run_conduction(args.n, args.steps, args.warmup, args.fp64_only)
     if not args.no_gravity:

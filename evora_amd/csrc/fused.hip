@@ -2,12 +2,13 @@
 //
 // k_fused runs the tile march of fused_core.h over one grid: one block per
 // (16 x TY tile, z chunk), with the tiles of each XCD contiguous; on 2-D
-// grids (nz = 1) one block per tile of the one plane.  k_cfl
+// grids (nz = 1), k_fused2d marches row strips along y (march2d.h).  k_cfl
 // gives the first step's CFL limit.  The file is compiled three times:
 // EVH_FUSED_PREC=64 (FP64, fast arithmetic), EVH_FUSED_PREC=32 (FP32-mixed)
 // and EVH_FUSED_EXACT=1 (FP64 with the reference's arithmetic, compiled
 // without FMA contraction).
 #include "fused_core.h"
+#include "march2d.h"
 
 #ifndef EVH_FUSED_PREC
 #define EVH_FUSED_PREC 64
@@ -23,8 +24,10 @@ constexpr bool kExact = EVH_FUSED_EXACT != 0;
 // two FP64 forms apart
 #if EVH_FUSED_EXACT
 #define K_FUSED k_fused_exact
+#define K_FUSED2D k_fused2d_exact
 #else
 #define K_FUSED k_fused
+#define K_FUSED2D k_fused2d
 #endif
 
 namespace evh {
@@ -34,16 +37,25 @@ using core::Geo;
 
 // WPE: waves per SIMD the register allocation must leave room for
 // (512 / WPE VGPRs per lane); the tuning knob next to the tile height TY.
-// D2: the 2-D form (nz = 1), the march without its z stage.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE, bool D2>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     K_FUSED(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
             S *__restrict__ tout) {
   __shared__ core::Smem<T, TY> sm;
   const int lin = xcd_tile(blockIdx.x, G.ntiles);
   if (lin < 0) return;  // padding block (whole block exits before any barrier)
-  core::march<T, S, SOLVER, LIMITER, TY, kExact, D2>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm,
-                                                     tout);
+  core::march<T, S, SOLVER, LIMITER, TY, kExact>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm, tout);
+}
+
+// 2-D grids (nz = 1): row strips marching along y (march2d.h)
+template <typename T, typename S, int SOLVER, int LIMITER, int WPE>
+__global__ void __launch_bounds__(core2d::NL) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+    K_FUSED2D(KParams P, core2d::Geo2 G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
+              S *__restrict__ tout) {
+  __shared__ core2d::Smem2<T> sm;
+  const int lin = xcd_tile(blockIdx.x, G.ntiles);
+  if (lin < 0) return;  // padding block (whole block exits before any barrier)
+  core2d::march2d<T, S, SOLVER, LIMITER, kExact>(P, G, qin, qout, dt, st, &st->vmin_bits, lin, sm, tout);
 }
 
 // primitive + CFL min over q without materialising w (first step, after
@@ -78,22 +90,43 @@ Geo make_geo(const evh_ctx *h, int ty) {
   return g;
 }
 
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE, bool D2 = false>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
 void launch_ty(evh_ctx *h, double dt) {
   const Geo g = make_geo(h, TY);
   const int blocks = xcd_blocks(g.ntiles);
-  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE, D2>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
+  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
+                     h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
+                     (S *)h->tbuf);
+}
+
+// the 2-D form: strips of 252 columns, chunks of 128 rows (the z chunk of the
+// tuning, when set), WPE waves per SIMD
+template <typename T, typename S, int SOLVER, int LIMITER, int WPE>
+void launch_2d(evh_ctx *h, double dt) {
+  core2d::Geo2 g;
+  g.nx = h->grid.nx;
+  g.ny = h->grid.ny;
+  g.n = h->grid.n();
+  g.ychunk = h->zchunk > 0 ? h->zchunk : 128;
+  if (g.ychunk > g.ny) g.ychunk = g.ny;
+  g.strips = (g.nx + core2d::NC - 1) / core2d::NC;
+  g.nchunks = (g.ny + g.ychunk - 1) / g.ychunk;
+  g.ntiles = g.strips * g.nchunks;
+  hipLaunchKernelGGL((K_FUSED2D<T, S, SOLVER, LIMITER, WPE>), dim3(xcd_blocks(g.ntiles)), dim3(core2d::NL), 0,
                      h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
                      (S *)h->tbuf);
 }
 
 // tuning variants: tile height 8 or 12, waves per SIMD (FP64: 2 or 3, FP32: 3
-// or 4); the 2-D form in one shape (16 x 12 tiles, the higher budget)
+// or 4); 2-D grids take the row-strip march in its one shape
 template <typename T, typename S, int SOLVER, int LIMITER>
 void launch_fused_t(evh_ctx *h, double dt) {
   constexpr int LO = sizeof(T) == 8 ? 2 : 3, HI = LO + 1;
   if (h->grid.nz == 1) {
-    launch_ty<T, S, SOLVER, LIMITER, 12, HI, true>(h, dt);
+    // 2-D: FP64 at 2, FP32 at 4 waves per SIMD, without spills (one more
+    // wave per SIMD spills 64 / 32 B per lane and measured 10% / 27% slower
+    // at 8192^2)
+    launch_2d<T, S, SOLVER, LIMITER, sizeof(T) == 8 ? 2 : 4>(h, dt);
     return;
   }
   const bool hi = h->wpe_hi;

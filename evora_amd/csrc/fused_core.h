@@ -27,12 +27,8 @@
 //       They sit where the interior waves have the least to do.
 // Template parameters: T compute type, S storage type of q (double/double
 // for the FP64 form, float/float for the FP32-mixed form), SOLVER, LIMITER,
-// TY the tile height (4, 8 or 12), D2 the 2-D form.
-//
-// The 2-D form (nz = 1, muscl.f90:70-88 and the x/y sweeps of :171-274) is
-// the same march without its z stage: one plane, no z slopes, z faces or
-// carried partial update; evo = dt2 (a1 dxinv + a2 dyinv), and plane 0 is
-// finished in S4 as (q + dtx*(Fx)) + dty*(Fy).
+// TY the tile height (4, 8 or 12).  2-D grids march along y instead
+// (march2d.h).
 #pragma once
 
 #include "evh_internal.h"
@@ -374,7 +370,7 @@ struct Smem {
 // the tile's new state; errors carry the box index above bit 40 of the cell.
 // With tout (single box only), the temperature of each new cell is stored
 // there too, for the conduction pass that follows the sweep.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT, bool D2 = false>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT>
 __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__restrict__ qin_all,
                                       S *__restrict__ qout_all, long long qoff, int box, double dt, Status *st,
                                       unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm,
@@ -458,32 +454,24 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   T rw2[5];
 #pragma unroll
   for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
-  if constexpr (D2) {
-    // the one plane: its q (the base of the update), w and second ring
-    T rr2;
-    load_cell<EXACT, S, T>(P, G, qin, gx, gy, 0, qc, wc, rc);
-    fetch_at(G, qin, r2_xy, pre2);
-    prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
-  } else {
-    load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
-    load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
-    fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
+  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
+  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
+  fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
+  {
     T rr2;
     fetch_at(G, qin, r2_xy + plane_offset(P, G, z0 - 1), pre2);
     prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
     fetch_at(G, qin, r2_xy + plane_offset(P, G, z0), pre2);
   }
 
-  for (int k = D2 ? 0 : z0 - 1; k <= (D2 ? 0 : z1); k++) {
-    const bool out_plane = D2 || (k >= z0 && k < z1);
+  for (int k = z0 - 1; k <= z1; k++) {
+    const bool out_plane = k >= z0 && k < z1;
     // ---- S1: primitive of plane k+1 (own column); rings of plane k to LDS.
     // Every lane forms and fetches (the ring wave's corner and idle lanes on
     // clamped coordinates), so that no branch makes a load wait for the
     // fetches issued after it.
-    if constexpr (!D2) {
-      prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
-      fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
-    }
+    prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
+    fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
     if (ring && r2x > -3) store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw2);
     if (owns_col) store_w<T, EA>(W, me, wc);
     __syncthreads();
@@ -496,18 +484,14 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     for (int m = 0; m < 5; m++) {
       sx[m] = Phys<EXACT>::template slope3<LIMITER>(W[m][me - 1], wc[m], W[m][me + 1]);
       sy[m] = Phys<EXACT>::template slope3<LIMITER>(W[m][me - EW], wc[m], W[m][me + EW]);
-      sz[m] = D2 ? T(0) : Phys<EXACT>::template slope3<LIMITER>(wm[m], wc[m], wp[m]);
+      sz[m] = Phys<EXACT>::template slope3<LIMITER>(wm[m], wc[m], wp[m]);
     }
     if (P.secondorder) {
       T a1[5], a2[5], a3[5];
       Phys<EXACT>::template predictor_dir<0>(P, wc, rc, sx, a1);
       Phys<EXACT>::template predictor_dir<1>(P, wc, rc, sy, a2);
-      if constexpr (D2) {
-        fast::evo5_2d(P, dt2, a1, a2, ev);  // muscl.f90:79-85
-      } else {
-        Phys<EXACT>::template predictor_dir<2>(P, wc, rc, sz, a3);
-        fast::evo5(P, dt2, a1, a2, a3, ev);  // muscl.f90:106-107
-      }
+      Phys<EXACT>::template predictor_dir<2>(P, wc, rc, sz, a3);
+      fast::evo5(P, dt2, a1, a2, a3, ev);  // muscl.f90:106-107
     } else {
 #pragma unroll
       for (int m = 0; m < 5; m++) { ev[m] = T(0); sx[m] = T(0); sy[m] = T(0); sz[m] = T(0); }
@@ -539,7 +523,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     // z face k-1/2 (interior lanes) and the end of plane k-1; the
     // scheduling barrier keeps this solve apart from the x/y solves
     __builtin_amdgcn_sched_barrier(0);
-    if (!D2 && !ring) {
+    if (!ring) {
       T zl[5], zr[5];
 #pragma unroll
       for (int m = 0; m < 5; m++) {
@@ -567,7 +551,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       for (int m = 0; m < 5; m++) CR[12 + m][tid] = zr[m];
     }
     }  // owns_col
-    if (!D2 && ring) {  // second-ring w of plane k+1, stored in S1 of plane k+1
+    if (ring) {  // second-ring w of plane k+1, stored in S1 of plane k+1
       T rr2;
       prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
       fetch_at(G, qin, r2_xy + plane_offset(P, G, k + 2), pre2);
@@ -636,19 +620,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       T part[6];
 #pragma unroll
       for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
-      if constexpr (D2) {  // the plane is finished: no z faces
-        if (inside) {
 #pragma unroll
-          for (int m = 0; m < 6; m++) o[m] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
-          oc = (long long)gx + (long long)G.nx * (long long)gy;
-#pragma unroll
-          for (int m = 0; m < 6; m++) qout[m * G.n + oc] = S(o[m]);
-          pend = true;
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
-      }
+      for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
     }
     // the end of plane k-1: primitive of the new state for the CFL minimum
     // of the next step, its first negative pressure, its temperature
@@ -657,21 +630,19 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
 #pragma unroll
       for (int m = 0; m < 6; m++) on[m] = T(S(o[m]));
       const int ep = Phys<EXACT>::q2w(P, on, wn, rn);
-      if (ep && !nerr) { nerr = ep; nerr_k = D2 ? 0 : k - 1; }
+      if (ep && !nerr) { nerr = ep; nerr_k = k - 1; }
       if constexpr (kSpeedKey) vmin = dmin(vmin, -fast::cfl_speed(P, wn, rn));
       else vmin = dmin(vmin, Phys<EXACT>::cfl_cell(P, wn, rn));
       if (tout) tout[oc] = S(Phys<EXACT>::temp(P, wn, rn));
       pend = false;
     }
     // rotate the column window
-    if constexpr (!D2) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) { wm[m] = wc[m]; wc[m] = wp[m]; }
-      rm = rc;
-      rc = rp;
+    for (int m = 0; m < 5; m++) { wm[m] = wc[m]; wc[m] = wp[m]; }
+    rm = rc;
+    rc = rp;
 #pragma unroll
-      for (int m = 0; m < 6; m++) qc[m] = qn[m];
-    }
+    for (int m = 0; m < 6; m++) qc[m] = qn[m];
   }
 
   if (err || nerr) {  // the lane's own column (clamped into the box) locates the error

@@ -41,14 +41,19 @@ def need_harnesses():
         pytest.skip("the reference harnesses are not built (make -C oracle ref ref-hip)")
 
 
-@pytest.mark.parametrize("case", ["hydro", "conduction"])
+@pytest.mark.parametrize("case", ["hydro", "conduction", "conduction_only"])
 def test_cpu_backend_is_the_reference(case):
+    """conduction_only: the reference's conduction test with hydro = .false.
+    (thermalstep, not hydrostep, then counts the steps for EVORA_MAXSTEPS)."""
     if case == "hydro":
         p = ics.blast_params(16, solver=2, cfl=0.4, tend=1e30)
         q0 = ics.blast(p)
-    else:
+    elif case == "conduction":
         p = ics.conduction_params(16, boundtype=3, tend=1e30)
         q0 = ics.conduction_blob(p)
+    else:
+        p = ics.thermaltest_params(tend=1e30)
+        q0 = ics.thermaltest(p)
     env = {"EVORA_BACKEND": "cpu", "EVORA_MAXSTEPS": "4"}
     qd, sd = _run(HIP_EXE, p, q0, 1000, env)
     qr, sr = _run(O.REF_PATH, p, q0, 4, {})

@@ -117,3 +117,44 @@ def test_dropin_evolution_on_device():
         assert got == pytest.approx(want, rel=1e-5)
     for row in rows:
         assert row[4] == pytest.approx(rows[0][4], rel=1e-5)
+
+
+def test_dropin_thermal_test_without_hydro():
+    """The reference's conduction test (hydro = .false., 100 cells, to t = 10)
+    through the Fortran drop-in: the driver calls neither hydrostep nor sweep
+    (main.f90:119,155), thermalstep counts the steps; the GPU run takes the
+    CPU run's 49 steps and lands within relative L1 1e-12 of it (T**2.5 of the
+    conduction limit comes from the device power)."""
+    p = ics.thermaltest_params()
+    q0 = ics.thermaltest(p)
+    qc, sc = run_dropin(p, q0, 100000, "cpu")
+    qh, sh = run_dropin(p, q0, 100000, "hip")
+    assert len(sc) == len(sh) == 49
+    assert float(sh[-1][2]) == float(sc[-1][2]) == p.tend
+    for c in range(6):
+        assert rel_l1(qh[..., c], qc[..., c]) <= 1e-12, c
+    assert np.array_equal(qh[..., :4], q0[..., :4])  # no hydro: mass and momentum untouched
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_dropin_2d_single_pass(exact):
+    """A 2-D grid (para/kevinhelmholtz's shear layer at 64^2, gamma 1.4)
+    through the Fortran drop-in takes the single-pass 2-D form: the exact form
+    equals the compiled reference bit for bit in q1-q5 and in every printed
+    t and dt, the fast form within relative L1 1e-10 after 12 steps."""
+    p = HydroParams(nx=64, ny=64, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.5, atomic=False, tend=12.0)
+    q0 = ics.kh_reference(p)
+    env = {"EVORA_MAXSTEPS": "12"}
+    if exact:
+        env["EVORA_EXACT"] = "1"
+    qc, sc = run_dropin(p, q0, 1000, "cpu", env)
+    qh, sh = run_dropin(p, q0, 1000, "hip", env)
+    assert len(sc) == len(sh) == 13
+    if exact:
+        assert sc == sh
+        for c in range(5):
+            assert np.array_equal(qh[..., c], qc[..., c]), c
+        assert rel_linf(qh[..., 5], qc[..., 5]) <= 1e-13
+    else:
+        for c in range(6):
+            assert rel_l1(qh[..., c], qc[..., c]) <= 1e-10, c

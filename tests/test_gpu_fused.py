@@ -168,8 +168,7 @@ def test_fused_smallest_grids(shape, bt):
 def test_fused_2d(solver, limiter, bt):
     """The single-pass 2-D form (nz = 1, muscl.f90:70-88 and the x/y sweeps):
     one step of the fast form to 1e-12 per component, the exact form equal to
-    the oracle bit for bit in q1-q5 over 3 steps, on a grid that is not a
-    multiple of the 16 x 12 tile."""
+    the oracle bit for bit in q1-q5 over 3 steps."""
     p = HydroParams(nx=37, ny=29, nz=1, solver=solver, limiter=limiter, boundtype=bt, cfl=0.4, atomic=True)
     d = p.derive()
     q0 = ics.random_smooth(p, seed=300 + 100 * solver + 10 * limiter + bt)
@@ -189,11 +188,14 @@ def test_fused_2d(solver, limiter, bt):
     assert rel_linf(q[..., 5], qo[..., 5]) <= 1e-13
 
 
+@pytest.mark.parametrize("shape", [(130, 70), (600, 300)])
 @pytest.mark.parametrize("precision", [hydro.FP64, hydro.FP32_MIXED])
-def test_fused_2d_run(precision):
-    """20 steps of the 2-D form on a 130 x 70 grid (partial tiles in x and y),
-    fast FP64 to relative L1 1e-10, FP32-mixed to 1e-5 in rho and p."""
-    p = HydroParams(nx=130, ny=70, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.4, atomic=True)
+def test_fused_2d_run(precision, shape):
+    """20 steps of the 2-D form, on one partial row strip and on three strips
+    of several row chunks: fast FP64 to relative L1 1e-10, FP32-mixed to
+    1e-5 in rho and p."""
+    nx, ny = shape
+    p = HydroParams(nx=nx, ny=ny, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.4, atomic=True)
     d = p.derive()
     q0 = ics.random_smooth(p, seed=77, amp=0.1)
     qo = q0.copy()

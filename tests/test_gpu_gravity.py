@@ -84,8 +84,13 @@ def test_potential_unequal_spacing():
         assert rel_linf(qg[..., c], qo[..., c]) <= 1e-12, c
 
 
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.FP32_MIXED])
 @pytest.mark.parametrize("shape", [(16, 12, 10), (16, 8, 4)])
-def test_potential_supercomoving_source(shape):
+def test_potential_supercomoving_source(shape, precision):
+    """The comoving source cosmo grav (rho/fb - 1), FP64 and from the FP32
+    state of the mixed form (against the oracle on the density rounded to
+    FP32); (16, 8, 4) takes this library's own transforms, whose x pass forms
+    the source as it loads."""
     nx, ny, nz = shape
     p = HydroParams(nx=nx, ny=ny, nz=nz, gravity=True, supercomoving=True, fb=0.17, atomic=True)
     d = p.derive()
@@ -93,7 +98,8 @@ def test_potential_supercomoving_source(shape):
     q = ics.random_smooth(p, seed=3)
     q[..., 0] = 0.1 + rng.random(q.shape[:3])
     cosmo = 0.37
-    assert rel_linf(_potential(d, q, cosmo), O.Oracle(d).gravpotential(q, cosmo)) <= 1e-12
+    qo = q if precision == hydro.FP64 else q.astype(np.float32).astype(np.float64)
+    assert rel_linf(_potential(d, q, cosmo, precision), O.Oracle(d).gravpotential(qo, cosmo)) <= 1e-12
 
 
 @pytest.mark.parametrize("n", [16, 12])
