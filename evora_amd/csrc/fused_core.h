@@ -588,7 +588,9 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
         // ring wave: the TY right-edge x faces (lanes 0..TY-1) and the 16
         // top-edge y faces (lanes TY..TY+15) in one solve, so that the wave
         // does not run two: a y face is the x face of the states with u and
-        // v exchanged (the solvers are symmetric under that exchange)
+        // v exchanged.  The solvers are bit-symmetric under that exchange:
+        // |u|^2 is formed symmetrically and every other term by its role
+        // (the FP32 build has no contraction).
         if (lane < TY + TX) {
           const bool isy = lane >= TY;
           T *fb = isy ? &YB[0][(lane - TY) + TX * TY] : &XB[0][TX + (TX + 1) * lane];

@@ -196,6 +196,15 @@ __device__ __forceinline__ T slope3(T wl, T wc, T wr) {
   }
 }
 
+// |u|^2 = (u_1^2 + u_2^2) + u_3^2 with the first two squares rounded on
+// their own: the same bits when u_1 and u_2 are exchanged (the ring wave
+// solves a y face as an x face), whatever contraction the compiler chooses
+// around it.  (The FP32 build of the march is compiled without contraction.)
+__device__ __forceinline__ double speed2(const double w[5]) {
+  return __builtin_fma(w[3], w[3], __dmul_rn(w[1], w[1]) + __dmul_rn(w[2], w[2]));
+}
+__device__ __forceinline__ float speed2(const float w[5]) { return (w[1] * w[1] + w[2] * w[2]) + w[3] * w[3]; }
+
 // flux (riemann.f90:157-205) of one state, with sq = p / rho**g1
 template <int DIR, typename T>
 __device__ __forceinline__ void flux(const KParams &P, const T w[5], T u2, T sq, T f[6]) {
@@ -245,8 +254,8 @@ __device__ __forceinline__ int hll(const KParams &P, const T wl[5], const T wr[5
   const T ar = fm::sqrt_pos(T(P.g) * wr[4] * rir);
   const T sl = dmin(wl[DIR] - al, wr[DIR] - ar);
   const T sr = dmax(wl[DIR] + al, wr[DIR] + ar);
-  const T u2l = wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3];
-  const T u2r = wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3];
+  const T u2l = speed2(wl);
+  const T u2r = speed2(wr);
   const bool left = T(0) < sl;
   const bool right = T(0) > sr;
   const bool mid = sl <= T(0) && T(0) <= sr;
@@ -292,16 +301,22 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   const T facl = wl[0] * (sl - wl[DIR]);
   const T facr = wr[0] * (sr - wr[DIR]);
   const T sm = (wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR]) * fm::rcp(facl - facr);
-  const int br = (T(0) < sl) ? 0
-               : (T(0) > sr) ? 1
-               : (sl < T(0) && T(0) < sm) ? 2
-               : (sm <= T(0) && T(0) < sr) ? 3 : -1;
+  // FP64: the reference's regions, with its error where none holds (a wave
+  // speed exactly 0: sl = 0 with sm > 0, or sr = 0 with sm <= 0).  FP32
+  // rounds onto such a speed far more often (a transonic face every few
+  // million in a Mach 17 cloud); there the star and outer fluxes coincide,
+  // so the FP32 form takes the star region and stops only on a NaN speed.
+  int br;
+  if constexpr (sizeof(T) == 8)
+    br = (T(0) < sl) ? 0 : (T(0) > sr) ? 1 : (sl < T(0) && T(0) < sm) ? 2 : (sm <= T(0) && T(0) < sr) ? 3 : -1;
+  else
+    br = (T(0) < sl) ? 0 : (T(0) > sr) ? 1 : (T(0) < sm) ? 2 : (sm <= T(0)) ? 3 : -1;
   if (br < 0 && !code) code = kErrHllc;
   const bool right = br == 1 || br == 3;
   T w[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) w[i] = right ? wr[i] : wl[i];
-  const T u2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+  const T u2 = speed2(w);
   const T sq = w[4] * rpow_g1(P, w[0]);  // p / rho**g1
   const T ud = w[DIR];
   const T mu = w[0] * ud;

@@ -211,3 +211,51 @@ def test_fused_2d_run(precision, shape):
         kin = lambda a: 0.5 * (a[..., 1] ** 2 + a[..., 2] ** 2 + a[..., 3] ** 2) / a[..., 0]
         assert rel_l1(q[..., 0], qo[..., 0]) <= 1e-5
         assert rel_l1(q[..., 4] - kin(q), qo[..., 4] - kin(qo)) <= 1e-5
+
+
+def _run_tuned(d, q0, prec, mode, tuning, steps):
+    with hydro.Hydro(d, precision=prec, mode=mode) as h:
+        h.set_tuning(*tuning)
+        h.upload_q(q0)
+        t = 0.0
+        for _ in range(steps):
+            t, dt, go = h.step(t, 1e30)
+        return h.download_q(), t
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp64_exact", "fp32_mixed"])
+@pytest.mark.parametrize("shape", [(24, 24, 24), (40, 36, 20)])
+def test_fused_result_independent_of_tiling(form, shape):
+    """The tile height, the waves-per-SIMD budget and the z chunk change
+    which lanes, call sites and waves form a cell's values (ring lane or
+    interior lane, first planes of a chunk or the plane ahead, a y face
+    solved by the ring wave as an x face with u and v exchanged): every
+    tuning must give the same bits after several steps of a blast."""
+    nx, ny, nz = shape
+    p = ics.blast_params(nx, solver=2, cfl=0.4, tend=1e30)
+    p.ny, p.nz = ny, nz
+    d = p.derive()
+    q0 = ics.blast(p)
+    prec = hydro.FP32_MIXED if form == "fp32_mixed" else hydro.FP64
+    mode = hydro.MODE_FUSED_EXACT if form == "fp64_exact" else hydro.MODE_FUSED
+    ref, t_ref = _run_tuned(d, q0, prec, mode, (12, 1, 0), 4)
+    for ty in (8, 12):
+        for hi in (0, 1):
+            for zc in (0, 8, 7):
+                q, t = _run_tuned(d, q0, prec, mode, (ty, hi, zc), 4)
+                assert t == t_ref, (ty, hi, zc)
+                diff = np.argwhere(np.any(q != ref, axis=-1))
+                assert len(diff) == 0, ((ty, hi, zc), len(diff), diff[:4].tolist())
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.FP32_MIXED])
+def test_fused_2d_independent_of_row_chunks(precision):
+    """2-D form: the row chunk (the z chunk of the tuning) sets where a strip
+    starts its march; every chunk gives the same bits."""
+    p = HydroParams(nx=300, ny=90, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.5, atomic=False)
+    d = p.derive()
+    q0 = ics.kh_reference(p)
+    ref, t_ref = _run_tuned(d, q0, precision, hydro.MODE_FUSED, (12, 1, 0), 4)
+    for zc in (7, 16, 45):
+        q, t = _run_tuned(d, q0, precision, hydro.MODE_FUSED, (12, 1, zc), 4)
+        assert t == t_ref and np.array_equal(q, ref), zc
