@@ -375,6 +375,7 @@ int evh_upload_q(evh_ctx *h, const double *q) {
   h->tv_valid = false;
   h->t_valid = false;
   h->w_valid = false;
+  h->rho_sums_valid = false;
   return EVH_OK;
 }
 

@@ -57,6 +57,9 @@ struct evh_ctx {
   void *spec = nullptr;
   double *sin2 = nullptr;
   double *rho_part = nullptr;
+  double *rho_sums = nullptr;   // the x pass's block totals of rho (power-of-two grids)
+  int rho_nsums = 0;
+  bool rho_sums_valid = false;  // they sum the state of the last potential, which an upload replaces
   void *plan_fwd = nullptr, *plan_inv = nullptr;  // hipfftHandle
   // power-of-two nz: the z transform is the pencil kernel of gravity.hip and
   // the plans transform x and y (2-D, per z plane)

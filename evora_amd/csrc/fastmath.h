@@ -165,6 +165,16 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
   return __builtin_fma(y, e, y);
 }
 
+// a*b + c.  FP32: one fused instruction, placed here by hand (the FP32
+// build of the march is compiled without contraction, so that every inlined
+// call site rounds alike).  FP64: the expression itself, contracted by the
+// fast build and left unfused by the exact build.
+template <typename T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return a * b + c;
+}
+
 // ---- float (hardware transcendentals; ~1-2 ulp)
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(x); }

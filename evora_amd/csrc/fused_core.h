@@ -86,7 +86,7 @@ struct Prim<float> {
     w[1] = q[1] * rinv;
     w[2] = q[2] * rinv;
     w[3] = q[3] * rinv;
-    const float m2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    const float m2 = fm::fmad(q[3], q[3], fm::fmad(q[2], q[2], q[1] * q[1]));
     const float kin = 0.5f * m2 * rinv;
     const bool fast_flow = kin > float(P.kin_frac64) * q[4];
     float pr;
@@ -502,23 +502,23 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        XB[5 + m][fxl] = (wc[m] - half * sx[m]) + ev[m];       // right state of the left face
-        XB[m][fxl + 1] = (wc[m] + half * sx[m]) + ev[m];       // left state of the right face
-        YB[5 + m][fyb] = (wc[m] - half * sy[m]) + ev[m];
-        YB[m][fyb + TX] = (wc[m] + half * sy[m]) + ev[m];
+        XB[5 + m][fxl] = fm::fmad(-half, sx[m], wc[m]) + ev[m];       // right state of the left face
+        XB[m][fxl + 1] = fm::fmad(half, sx[m], wc[m]) + ev[m];       // left state of the right face
+        YB[5 + m][fyb] = fm::fmad(-half, sy[m], wc[m]) + ev[m];
+        YB[m][fyb + TX] = fm::fmad(half, sy[m], wc[m]) + ev[m];
       }
     } else if (side == 0) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = (wc[m] + half * sx[m]) + ev[m];
+      for (int m = 0; m < 5; m++) XB[m][(TX + 1) * cy] = fm::fmad(half, sx[m], wc[m]) + ev[m];
     } else if (side == 1) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = (wc[m] - half * sx[m]) + ev[m];
+      for (int m = 0; m < 5; m++) XB[5 + m][TX + (TX + 1) * cy] = fm::fmad(-half, sx[m], wc[m]) + ev[m];
     } else if (side == 2) {
 #pragma unroll
-      for (int m = 0; m < 5; m++) YB[m][cx] = (wc[m] + half * sy[m]) + ev[m];
+      for (int m = 0; m < 5; m++) YB[m][cx] = fm::fmad(half, sy[m], wc[m]) + ev[m];
     } else {
 #pragma unroll
-      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = (wc[m] - half * sy[m]) + ev[m];
+      for (int m = 0; m < 5; m++) YB[5 + m][cx + TX * TY] = fm::fmad(-half, sy[m], wc[m]) + ev[m];
     }
     // z face k-1/2 (interior lanes) and the end of plane k-1; the
     // scheduling barrier keeps this solve apart from the x/y solves
@@ -527,8 +527,8 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       T zl[5], zr[5];
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        zl[m] = (wc[m] - half * sz[m]) + ev[m];
-        zr[m] = (wc[m] + half * sz[m]) + ev[m];
+        zl[m] = fm::fmad(-half, sz[m], wc[m]) + ev[m];
+        zr[m] = fm::fmad(half, sz[m], wc[m]) + ev[m];
       }
       if (k >= z0) {
         T zrp[5], fz[6];
@@ -538,7 +538,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
         if (e && inside && !err) { err = e; err_k = k; }
         if (k - 1 >= z0 && inside) {
 #pragma unroll
-          for (int m = 0; m < 6; m++) o[m] = CR[m][tid] + dtz * (CR[6 + m][tid] - fz[m]);
+          for (int m = 0; m < 6; m++) o[m] = fm::fmad(dtz, CR[6 + m][tid] - fz[m], CR[m][tid]);
           oc = (long long)gx + (long long)G.nx * ((long long)gy + (long long)G.ny * (k - 1));
 #pragma unroll
           for (int m = 0; m < 6; m++) qout[m * G.n + oc] = S(o[m]);
@@ -621,9 +621,9 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
       const int fxl = cx + (TX + 1) * cy, fyb = cx + TX * cy;
       T part[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][fxl] - XB[m][fxl + 1]);
+      for (int m = 0; m < 6; m++) part[m] = fm::fmad(dtx, XB[m][fxl] - XB[m][fxl + 1], qc[m]);
 #pragma unroll
-      for (int m = 0; m < 6; m++) CR[m][tid] = part[m] + dty * (YB[m][fyb] - YB[m][fyb + TX]);
+      for (int m = 0; m < 6; m++) CR[m][tid] = fm::fmad(dty, YB[m][fyb] - YB[m][fyb + TX], part[m]);
     }
     // the end of plane k-1: primitive of the new state for the CFL minimum
     // of the next step, its first negative pressure, its temperature

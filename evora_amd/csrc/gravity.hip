@@ -127,6 +127,26 @@ __global__ void __launch_bounds__(kSumBlocks) k_rho_mean(double *part, double n3
   if (threadIdx.x == 0) part[kSumBlocks] = red[0] / n3;  // meanrho = sum / n3
 }
 
+// the mean density from the block totals of the previous call's x pass
+// (Neumaier sums per thread, then a pairwise tree)
+__global__ void __launch_bounds__(kSumBlocks) k_rho_mean_from(const double *__restrict__ sums, int nsums, double n3,
+                                                            double *part) {
+  __shared__ double red[kSumBlocks];
+  double s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nsums; i += kSumBlocks) {
+    const double v = sums[i], t = s + v;
+    c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  red[threadIdx.x] = s + c;
+  __syncthreads();
+  for (int w = kSumBlocks / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[kSumBlocks] = red[0] / n3;
+}
+
 // ---- the source of gravitation.f90:33-42
 template <typename S>
 __global__ void __launch_bounds__(256) k_grav_source(const S *__restrict__ q, long long n, double *__restrict__ phi,
@@ -225,6 +245,7 @@ struct PencilTile {
   const void *rho;    // q(1) of the first row (storage type)
   double *phi;        // phi of the first row
   double mean, cosmo_grav;
+  double *racc;       // k_xfwd: this thread's compensated sum of the densities it reads (sum, error)
 };
 
 // the source of gravpotential (gravitation.f90:26-53) of one density, as
@@ -315,8 +336,18 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
         x = make_double2(0.0, 0.0);
         if (t < pt.cols) {
           const S *r0 = (const S *)pt.rho + (long long)(2 * t) * (1 << LN);
-          x.x = grav_source<SC>(double(r0[i]), G, pt.mean, pt.cosmo_grav);
-          x.y = grav_source<SC>(double(r0[i + (1 << LN)]), G, pt.mean, pt.cosmo_grav);
+          const double ra = double(r0[i]), rb = double(r0[i + (1 << LN)]);
+          x.x = grav_source<SC>(ra, G, pt.mean, pt.cosmo_grav);
+          x.y = grav_source<SC>(rb, G, pt.mean, pt.cosmo_grav);
+          if constexpr (!SC) {  // Neumaier sums of the densities, for the next call's mean
+            const double v[2] = {ra, rb};
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+              const double t2 = pt.racc[0] + v[q];
+              pt.racc[1] += fabs(pt.racc[0]) >= fabs(v[q]) ? (pt.racc[0] - t2) + v[q] : (v[q] - t2) + pt.racc[0];
+              pt.racc[0] = t2;
+            }
+          }
         }
       } else if (SRC == kFromHalves) {
         // Z(k) = A(k) + i B(k) over the whole row, A and B Hermitian; the
@@ -435,10 +466,13 @@ k_pencils(double2 *__restrict__ spec, Pencils pc, const double2 *__restrict__ tw
 // B(k) = (Z(k) - conj Z(n-k)) / 2i, k <= n/2, go to the two rows of the half
 // spectrum.  This is the real-to-complex transform of each row, two rows per
 // complex transform.  SC: the supercomoving source.
+// Without SC the pass also sums the densities it reads, one total per block
+// into rho_sum[blockIdx.x], from which the next call takes its mean.
 template <typename S, int LN, bool SC>
 __global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
 k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, long long npairs,
-       const double2 *__restrict__ tw, GParams G, double cosmo_grav, const double *__restrict__ mean_part) {
+       const double2 *__restrict__ tw, GParams G, double cosmo_grav, const double *__restrict__ mean_part,
+       double *__restrict__ rho_sum) {
   constexpr int N = 1 << LN, TK = 1 << (kPencilLog - LN), NH = N / 2 + 1;
   using TI = TileIndex<LN, true>;
   __shared__ double2 X[tile_elems<true>()];
@@ -453,6 +487,8 @@ k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, l
   pt.phi = nullptr;
   pt.mean = SC ? 0.0 : mean_part[kSumBlocks];
   pt.cosmo_grav = cosmo_grav;
+  double acc[2] = {0.0, 0.0};
+  pt.racc = acc;
   pencil_stages<-1, LN, 0, kFromSource, kToLds, S, true, SC>(pt, pitch, tw, nullptr, G, 0, 0, 0);
   for (int e = threadIdx.x; e < TK * NH; e += kPencilThreads) {
     const int t = e / NH, k = e - t * NH;
@@ -466,6 +502,17 @@ k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, l
       __builtin_nontemporal_store(b.x, &ra[pitch].x);
       __builtin_nontemporal_store(b.y, &ra[pitch].y);
     }
+  }
+  if constexpr (!SC) {  // the block's total of rho, through the tile's LDS once every lane has read it
+    double *red = (double *)X;
+    __syncthreads();
+    red[threadIdx.x] = acc[0] + acc[1];
+    __syncthreads();
+    for (int w = kPencilThreads / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) rho_sum[blockIdx.x] = red[0];
   }
 }
 
@@ -675,19 +722,26 @@ void launch_pencils(const Pencils &pc, int nouter, hipStream_t st, double2 *spec
 // tile of row pairs
 template <typename S, int LN = 1>
 void launch_xfwd(int ln, const S *rho, double2 *spec, long long pitch, long long npairs, hipStream_t st,
-                 const double2 *tw, const GParams &gp, double cosmo_grav, const double *mean_part) {
+                 const double2 *tw, const GParams &gp, double cosmo_grav, const double *mean_part, double *rho_sum) {
   if constexpr (LN <= kPencilLog) {
     constexpr int TK = 1 << (kPencilLog - LN);
     const dim3 grid((unsigned)((npairs + TK - 1) / TK));
     if (ln == LN && gp.supercomoving)
       hipLaunchKernelGGL((k_xfwd<S, LN, true>), grid, dim3(kPencilThreads), 0, st, rho, spec, pitch, npairs, tw, gp,
-                         cosmo_grav, mean_part);
+                         cosmo_grav, mean_part, rho_sum);
     else if (ln == LN)
       hipLaunchKernelGGL((k_xfwd<S, LN, false>), grid, dim3(kPencilThreads), 0, st, rho, spec, pitch, npairs, tw, gp,
-                         cosmo_grav, mean_part);
+                         cosmo_grav, mean_part, rho_sum);
     else
-      launch_xfwd<S, LN + 1>(ln, rho, spec, pitch, npairs, st, tw, gp, cosmo_grav, mean_part);
+      launch_xfwd<S, LN + 1>(ln, rho, spec, pitch, npairs, st, tw, gp, cosmo_grav, mean_part, rho_sum);
   }
+}
+// blocks of the x pass: the tiles of row pairs
+int xfwd_blocks(int nx, long long npairs) {
+  int ln = 0;
+  while ((1 << ln) < nx) ln++;
+  const long long tk = 1LL << (kPencilLog - ln);
+  return (int)((npairs + tk - 1) / tk);
 }
 template <int LN = 1>
 void launch_xinv(int ln, const double2 *spec, double *phi, long long pitch, long long npairs, hipStream_t st,
@@ -736,6 +790,12 @@ hipError_t gravity_init(evh_ctx *h) {
   if ((e = hipMalloc(&h->sin2, sizeof(double) * (c.nx + c.ny + c.nz))) != hipSuccess) return e;
   if ((e = hipMalloc(&h->rho_part, sizeof(double) * (kSumBlocks + 1))) != hipSuccess) return e;
   h->dev_bytes += (long long)sizeof(double) * (n + 2 * nspec + c.nx + c.ny + c.nz + kSumBlocks + 1);
+  if (xyz) {  // the x pass's block totals of rho
+    h->rho_nsums = xfwd_blocks(c.nx, (long long)c.ny * c.nz / 2);
+    if ((e = hipMalloc(&h->rho_sums, sizeof(double) * h->rho_nsums)) != hipSuccess) return e;
+    h->dev_bytes += (long long)sizeof(double) * h->rho_nsums;
+  }
+  h->rho_sums_valid = false;
   // the sin^2 factors of multigreen, evaluated on the host like the reference
   std::vector<double> tab;
   const int ns[3] = {c.nx, c.ny, c.nz}, nyq[3] = {c.nyqx, c.nyqy, c.nyqz};
@@ -796,6 +856,10 @@ void gravity_release(evh_ctx *h) {
   if (h->spec) hipFree(h->spec);
   if (h->sin2) hipFree(h->sin2);
   if (h->rho_part) hipFree(h->rho_part);
+  if (h->rho_sums) hipFree(h->rho_sums);
+  h->rho_sums = nullptr;
+  h->rho_nsums = 0;
+  h->rho_sums_valid = false;
   if (h->twiddle) hipFree(h->twiddle);
   h->twiddle = nullptr;
   h->plan_fwd = h->plan_inv = nullptr;
@@ -808,7 +872,17 @@ hipError_t launch_gravpotential(evh_ctx *h, double cosmo) {
   const long long n = h->grid.n();
   const GParams gp = make_gparams(h);
   const bool f32 = h->cfg.precision == EVH_FP32_MIXED;
-  if (!gp.supercomoving) {
+  // The mean density only sets the (0,0,0) mode of the source, which the
+  // Green's function zeroes; it keeps the transform's input small.  The
+  // x pass of this library's transforms sums the densities it reads, and a
+  // later call takes the mean of that earlier state (the same to rounding
+  // while the flow conserves mass); the first call after an upload sums the
+  // density first (gravitation.f90:39-42).
+  const bool reuse = h->xyz_pencils && h->rho_sums_valid;
+  if (!gp.supercomoving && reuse) {
+    hipLaunchKernelGGL(k_rho_mean_from, dim3(1), dim3(kSumBlocks), 0, h->stream, h->rho_sums, h->rho_nsums, gp.n3,
+                       h->rho_part);
+  } else if (!gp.supercomoving) {
     if (f32)
       hipLaunchKernelGGL(k_rho_partials<float>, dim3(kSumBlocks), dim3(256), 0, h->stream,
                          (const float *)h->qbuf[h->cur], n, h->rho_part);
@@ -830,10 +904,11 @@ hipError_t launch_gravpotential(evh_ctx *h, double cosmo) {
     while ((1 << lnx) < nx) lnx++;
     if (f32)
       launch_xfwd<float>(lnx, (const float *)h->qbuf[h->cur], spec, h->spec_pitch, npairs, h->stream, twx, gp,
-                         cosmo_grav, h->rho_part);
+                         cosmo_grav, h->rho_part, h->rho_sums);
     else
       launch_xfwd<double>(lnx, (const double *)h->qbuf[h->cur], spec, h->spec_pitch, npairs, h->stream, twx, gp,
-                          cosmo_grav, h->rho_part);
+                          cosmo_grav, h->rho_part, h->rho_sums);
+    h->rho_sums_valid = !gp.supercomoving;
     launch_pencils<kPencilFwdY>(py, nz, h->stream, spec, twy, h->sin2, gp, nx, ny, nz);
     launch_pencils<kPencilGreenZ>(pz, ny, h->stream, spec, twz, h->sin2, gp, nx, ny, nz);
     launch_pencils<kPencilInvY>(py, nz, h->stream, spec, twy, h->sin2, gp, nx, ny, nz);

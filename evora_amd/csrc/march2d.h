@@ -155,18 +155,18 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
       // (w - 0.5 s) + evo on the low side
       if (face) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) XB[m][l] = (wc[m] + half * sx[m]) + ev[m];
+        for (int m = 0; m < 5; m++) XB[m][l] = fm::fmad(half, sx[m], wc[m]) + ev[m];
       }
       if (l >= 2) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) XB[5 + m][l - 1] = (wc[m] - half * sx[m]) + ev[m];
+        for (int m = 0; m < 5; m++) XB[5 + m][l - 1] = fm::fmad(-half, sx[m], wc[m]) + ev[m];
       }
       if (inner) {
         T yb[5], yt[5];
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-          yb[m] = (wc[m] - half * sy[m]) + ev[m];
-          yt[m] = (wc[m] + half * sy[m]) + ev[m];
+          yb[m] = fm::fmad(-half, sy[m], wc[m]) + ev[m];
+          yt[m] = fm::fmad(half, sy[m], wc[m]) + ev[m];
         }
         if (k >= y0) {
           T fy[6];
@@ -174,7 +174,7 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
           if (e && inside && !err) { err = e; err_k = k; }
           if (k - 1 >= y0 && inside) {
 #pragma unroll
-            for (int m = 0; m < 6; m++) o[m] = part[m] + dty * (fyb[m] - fy[m]);
+            for (int m = 0; m < 6; m++) o[m] = fm::fmad(dty, fyb[m] - fy[m], part[m]);
             oc = (long long)gx + (long long)G.nx * (k - 1);
 #pragma unroll
             for (int m = 0; m < 6; m++) qout[m * G.n + oc] = S(o[m]);
@@ -207,7 +207,7 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
     // ---- S4: q(k) + dtx*(Fx) of row k; the end of row k-1
     if (out_row && inner) {
 #pragma unroll
-      for (int m = 0; m < 6; m++) part[m] = qc[m] + dtx * (XB[m][l - 1] - XB[m][l]);
+      for (int m = 0; m < 6; m++) part[m] = fm::fmad(dtx, XB[m][l - 1] - XB[m][l], qc[m]);
     }
     if (pend) {
       T on[6], wn[5], rn;

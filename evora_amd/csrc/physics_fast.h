@@ -143,8 +143,8 @@ __device__ __forceinline__ void predictor_dir(const KParams &P, const float w[5]
                                               float a[5]) {
 #pragma unroll
   for (int i = 0; i < 5; i++) a[i] = w[D + 1] * s[i];
-  a[4] = a[4] + float(P.g) * w[4] * s[D + 1];
-  a[D + 1] = a[D + 1] + s[4] * rinv;
+  a[4] = fm::fmad(float(P.g) * w[4], s[D + 1], a[4]);
+  a[D + 1] = fm::fmad(s[4], rinv, a[D + 1]);
 }
 
 // evo (muscl.f90:106-107) from the three directions' predictor terms:
@@ -167,13 +167,13 @@ __device__ __forceinline__ void evo5_2d(const KParams &P, float dt2, const float
                                         float ev[5]) {
   const float ex = dt2 * float(P.dxinv), ey = dt2 * float(P.dyinv);
 #pragma unroll
-  for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey;
+  for (int m = 0; m < 5; m++) ev[m] = fm::fmad(a1[m], ex, a2[m] * ey);
 }
 __device__ __forceinline__ void evo5(const KParams &P, float dt2, const float a1[5], const float a2[5],
                                      const float a3[5], float ev[5]) {
   const float ex = dt2 * float(P.dxinv), ey = dt2 * float(P.dyinv), ez = dt2 * float(P.dzinv);
 #pragma unroll
-  for (int m = 0; m < 5; m++) ev[m] = a1[m] * ex + a2[m] * ey + a3[m] * ez;
+  for (int m = 0; m < 5; m++) ev[m] = fm::fmad(a1[m], ex, fm::fmad(a2[m], ey, a3[m] * ez));
 }
 
 // muscl.f90:115-169, one component.  minmod and monotonized central in
@@ -203,7 +203,7 @@ __device__ __forceinline__ T slope3(T wl, T wc, T wr) {
 __device__ __forceinline__ double speed2(const double w[5]) {
   return __builtin_fma(w[3], w[3], __dmul_rn(w[1], w[1]) + __dmul_rn(w[2], w[2]));
 }
-__device__ __forceinline__ float speed2(const float w[5]) { return (w[1] * w[1] + w[2] * w[2]) + w[3] * w[3]; }
+__device__ __forceinline__ float speed2(const float w[5]) { return __builtin_fmaf(w[3], w[3], w[1] * w[1] + w[2] * w[2]); }
 
 // flux (riemann.f90:157-205) of one state, with sq = p / rho**g1
 template <int DIR, typename T>
@@ -212,8 +212,8 @@ __device__ __forceinline__ void flux(const KParams &P, const T w[5], T u2, T sq,
   const T mu = w[0] * ud;
   f[0] = mu;
 #pragma unroll
-  for (int i = 1; i <= 3; i++) f[i] = (i == DIR) ? (mu * ud + w[4]) : (mu * w[i]);
-  f[4] = ud * (T(0.5) * w[0] * u2 + T(P.gg1) * w[4]);
+  for (int i = 1; i <= 3; i++) f[i] = (i == DIR) ? fm::fmad(mu, ud, w[4]) : (mu * w[i]);
+  f[4] = ud * fm::fmad(T(0.5) * w[0], u2, T(P.gg1) * w[4]);
   f[5] = ud * sq;
 }
 
@@ -224,7 +224,7 @@ __device__ __forceinline__ void state(const KParams &P, const T w[5], T u2, T sq
   q[1] = w[0] * w[1];
   q[2] = w[0] * w[2];
   q[3] = w[0] * w[3];
-  q[4] = T(0.5) * w[0] * u2 + w[4] * T(P.g1inv);
+  q[4] = fm::fmad(T(0.5) * w[0], u2, w[4] * T(P.g1inv));
   q[5] = sq;
 }
 
@@ -271,7 +271,9 @@ __device__ __forceinline__ int hll(const KParams &P, const T wl[5], const T wr[5
   const T slsr = sl * sr;
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    const T fm_ = (sr * fl[i] - sl * fr[i] + slsr * (qr[i] - ql[i])) * inv;
+    T fm_;
+    if constexpr (sizeof(T) == 4) fm_ = fm::fmad(slsr, qr[i] - ql[i], fm::fmad(sr, fl[i], -(sl * fr[i]))) * inv;
+    else fm_ = (sr * fl[i] - sl * fr[i] + slsr * (qr[i] - ql[i])) * inv;
     f[i] = left ? fl[i] : (right ? fr[i] : fm_);
   }
   return code;
@@ -300,7 +302,11 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   const T sr = dmax(wl[DIR] + al, wr[DIR] + ar);
   const T facl = wl[0] * (sl - wl[DIR]);
   const T facr = wr[0] * (sr - wr[DIR]);
-  const T sm = (wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR]) * fm::rcp(facl - facr);
+  T sm;
+  if constexpr (sizeof(T) == 4)
+    sm = fm::fmad(-facr, wr[DIR], fm::fmad(facl, wl[DIR], wr[4] - wl[4])) * fm::rcp(facl - facr);
+  else
+    sm = (wr[4] - wl[4] + facl * wl[DIR] - facr * wr[DIR]) * fm::rcp(facl - facr);
   // FP64: the reference's regions, with its error where none holds (a wave
   // speed exactly 0: sl = 0 with sm > 0, or sr = 0 with sm <= 0).  FP32
   // rounds onto such a speed far more often (a transonic face every few
@@ -320,10 +326,10 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   const T sq = w[4] * rpow_g1(P, w[0]);  // p / rho**g1
   const T ud = w[DIR];
   const T mu = w[0] * ud;
-  const T energy = T(0.5) * w[0] * u2 + w[4] * T(P.g1inv);
+  const T energy = fm::fmad(T(0.5) * w[0], u2, w[4] * T(P.g1inv));
   f[0] = mu;
 #pragma unroll
-  for (int i = 1; i <= 3; i++) f[i] = (i == DIR) ? (mu * ud + w[4]) : (mu * w[i]);
+  for (int i = 1; i <= 3; i++) f[i] = (i == DIR) ? fm::fmad(mu, ud, w[4]) : (mu * w[i]);
   f[4] = ud * (energy + w[4]);  // ud (0.5 rho u2 + gg1 p) with gg1 = 1 + 1/g1
   f[5] = ud * sq;
   const T s = right ? sr : sl;
@@ -335,13 +341,20 @@ __device__ __forceinline__ int hllc(const KParams &P, const T wl[5], const T wr[
   const T qx0 = w[0] * r;
   T dq[6];
   dq[0] = qx0 - w[0];
+  if constexpr (sizeof(T) == 4) {
 #pragma unroll
-  for (int i = 1; i <= 3; i++) dq[i] = ((i == DIR) ? sm : w[i]) * qx0 - w[0] * w[i];
-  dq[4] = r * energy + (sm - ud) * (qx0 * sm + w[4] * isd) - energy;
-  dq[5] = r * sq - sq;
+    for (int i = 1; i <= 3; i++) dq[i] = fm::fmad((i == DIR) ? sm : w[i], qx0, -(w[0] * w[i]));
+    dq[4] = fm::fmad(r, energy, fm::fmad(sm - ud, fm::fmad(qx0, sm, w[4] * isd), -energy));
+    dq[5] = fm::fmad(r, sq, -sq);
+  } else {
+#pragma unroll
+    for (int i = 1; i <= 3; i++) dq[i] = ((i == DIR) ? sm : w[i]) * qx0 - w[0] * w[i];
+    dq[4] = r * energy + (sm - ud) * (qx0 * sm + w[4] * isd) - energy;
+    dq[5] = r * sq - sq;
+  }
   const T ss = star ? s : T(0);
 #pragma unroll
-  for (int i = 0; i < 6; i++) f[i] = f[i] + ss * dq[i];
+  for (int i = 0; i < 6; i++) f[i] = fm::fmad(ss, dq[i], f[i]);
   return code;
 }
 
