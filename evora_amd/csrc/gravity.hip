@@ -245,7 +245,7 @@ struct PencilTile {
   const void *rho;    // q(1) of the first row (storage type)
   double *phi;        // phi of the first row
   double mean, cosmo_grav;
-  double *racc;       // k_xfwd: this thread's compensated sum of the densities it reads (sum, error)
+  double *racc;       // k_xfwd: this thread's sum of the densities it reads (16 values at nx = 512)
 };
 
 // the source of gravpotential (gravitation.f90:26-53) of one density, as
@@ -339,15 +339,7 @@ __device__ __forceinline__ void pencil_stage(const PencilTile &pt, long long s_e
           const double ra = double(r0[i]), rb = double(r0[i + (1 << LN)]);
           x.x = grav_source<SC>(ra, G, pt.mean, pt.cosmo_grav);
           x.y = grav_source<SC>(rb, G, pt.mean, pt.cosmo_grav);
-          if constexpr (!SC) {  // Neumaier sums of the densities, for the next call's mean
-            const double v[2] = {ra, rb};
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-              const double t2 = pt.racc[0] + v[q];
-              pt.racc[1] += fabs(pt.racc[0]) >= fabs(v[q]) ? (pt.racc[0] - t2) + v[q] : (v[q] - t2) + pt.racc[0];
-              pt.racc[0] = t2;
-            }
-          }
+          if constexpr (!SC) pt.racc[0] += ra + rb;  // the densities' sum, for the next call's mean
         }
       } else if (SRC == kFromHalves) {
         // Z(k) = A(k) + i B(k) over the whole row, A and B Hermitian; the
@@ -467,7 +459,7 @@ k_pencils(double2 *__restrict__ spec, Pencils pc, const double2 *__restrict__ tw
 // spectrum.  This is the real-to-complex transform of each row, two rows per
 // complex transform.  SC: the supercomoving source.
 // Without SC the pass also sums the densities it reads, one total per block
-// into rho_sum[blockIdx.x], from which the next call takes its mean.
+// into rho_sum, from which the next call takes its mean.
 template <typename S, int LN, bool SC>
 __global__ void __launch_bounds__(kPencilThreads) __attribute__((amdgpu_waves_per_eu(kPencilWaves, kPencilWaves)))
 k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, long long npairs,
@@ -487,7 +479,7 @@ k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, l
   pt.phi = nullptr;
   pt.mean = SC ? 0.0 : mean_part[kSumBlocks];
   pt.cosmo_grav = cosmo_grav;
-  double acc[2] = {0.0, 0.0};
+  double acc[1] = {0.0};
   pt.racc = acc;
   pencil_stages<-1, LN, 0, kFromSource, kToLds, S, true, SC>(pt, pitch, tw, nullptr, G, 0, 0, 0);
   for (int e = threadIdx.x; e < TK * NH; e += kPencilThreads) {
@@ -503,16 +495,18 @@ k_xfwd(const S *__restrict__ rho, double2 *__restrict__ spec, long long pitch, l
       __builtin_nontemporal_store(b.y, &ra[pitch].y);
     }
   }
-  if constexpr (!SC) {  // the block's total of rho, through the tile's LDS once every lane has read it
-    double *red = (double *)X;
+  if constexpr (!SC) {  // the block's total of rho: a butterfly per wave, then the waves' totals
+    __shared__ double wsum[kPencilThreads / 64];
+    double v = acc[0];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
-    red[threadIdx.x] = acc[0] + acc[1];
-    __syncthreads();
-    for (int w = kPencilThreads / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < kPencilThreads / 64; i++) t += wsum[i];
+      rho_sum[blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) rho_sum[blockIdx.x] = red[0];
   }
 }
 
@@ -736,7 +730,7 @@ void launch_xfwd(int ln, const S *rho, double2 *spec, long long pitch, long long
       launch_xfwd<S, LN + 1>(ln, rho, spec, pitch, npairs, st, tw, gp, cosmo_grav, mean_part, rho_sum);
   }
 }
-// blocks of the x pass: the tiles of row pairs
+// blocks of the x pass (the tiles of row pairs), one density total each
 int xfwd_blocks(int nx, long long npairs) {
   int ln = 0;
   while ((1 << ln) < nx) ln++;
