@@ -63,7 +63,11 @@ __device__ __forceinline__ long long src_cell(const CGeo &G, bool per, int gx, i
   gx = min(max(gx, -2), G.nx + 1);
   gy = min(max(gy, -2), G.ny + 1);
   gz = min(max(gz, -2), G.nz + 1);
-  const int sx = ghost_src(gx, G.nx, per), sy = ghost_src(gy, G.ny, per), sz = ghost_src(gz, G.nz, per);
+  // an axis of one cell (1-D and 2-D grids) has that cell as every ghost:
+  // the two-layer rules would name cells -1 or 1 of it
+  const int sx = G.nx == 1 ? 0 : ghost_src(gx, G.nx, per);
+  const int sy = G.ny == 1 ? 0 : ghost_src(gy, G.ny, per);
+  const int sz = G.nz == 1 ? 0 : ghost_src(gz, G.nz, per);
   return (long long)sx + (long long)G.nx * ((long long)sy + (long long)G.ny * sz);
 }
 
