@@ -457,6 +457,34 @@ def copy_probe():
     log(f"copied {a.numel() * 8} bytes three times")
 
 
+def stream_probe():
+    """The HBM rate plain streaming kernels reach on this part: a device copy
+    and a triad c = a + 1.5 b over 2 GiB FP64 arrays, timed with HIP events
+    (mean of 10 after a warm-up), as the practical ceiling beside the
+    8 TB/s specification."""
+    import torch
+
+    n = 2 ** 28
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.rand(n, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    out = {}
+    for name, fn, nbytes in (("copy", lambda: b.copy_(a), 2 * 8 * n),
+                             ("triad", lambda: torch.add(a, b, alpha=1.5, out=c), 3 * 8 * n)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        out[name] = dict(ms=ms, GBs=nbytes / (ms * 1e-3) / 1e9)
+        log(f"{name}: {ms:.3f} ms, {out[name]['GBs']:.0f} GB/s ({out[name]['GBs'] / HBM_PEAK_GBS:.2f} of the peak)")
+    print(json.dumps({"stream_probe": out, "bytes_per_array": 8 * n}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,11 +507,15 @@ def main():
     ap.add_argument("--gravity-only", action="store_true",
                     help="run only the self-gravity form and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
+    ap.add_argument("--stream-probe", action="store_true", help="time a plain copy and a triad (HBM ceiling)")
     ap.add_argument("--batch-boxes", type=int, default=48)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--autotune-log", default=None,
                     help="file to write the batched form's autotuner table to (the chosen shape is in the JSON)")
     args = ap.parse_args()
+    if args.stream_probe:
+        stream_probe()
+        return
     if args.copy_probe:
         copy_probe()
         return
