@@ -15,6 +15,9 @@
 // middle frequencies whose sines differ by the rounding of the
 // single-precision pi of global.f90:98.  The potential stays in FP64 in the
 // mixed form too: the force is a difference of neighbouring potentials.
+// The mean density only sets the (0,0,0) mode, which the Green's function
+// zeroes; on power-of-two grids the x pass sums rho as it reads it and the
+// next call takes its mean from those sums (an upload invalidates them).
 //
 // Transforms.  When nz is a power of two, the z transform is the pencil
 // kernel below: a block takes a tile of consecutive kx columns of one y row,

@@ -228,19 +228,6 @@ __device__ __forceinline__ void state(const KParams &P, const T w[5], T u2, T sq
   q[5] = sq;
 }
 
-// statex (riemann.f90:241-288): HLLC star state of w for wave speed s
-template <int DIR, typename T>
-__device__ __forceinline__ void statex(const KParams &P, const T w[5], T rinv, T u2, T sq, T s, T sm, T qx[6]) {
-  const T ud = w[DIR];
-  const T sud = s - ud;
-  qx[0] = w[0] * sud * fm::rcp(s - sm);
-#pragma unroll
-  for (int i = 1; i <= 3; i++) qx[i] = ((i == DIR) ? sm : w[i]) * qx[0];
-  const T energy = T(0.5) * w[0] * u2 + w[4] * T(P.g1inv);
-  qx[4] = qx[0] * (energy * rinv + (sm - ud) * (sm + w[4] * rinv * fm::rcp(sud)));
-  qx[5] = qx[0] * sq * rinv;  // qx1 * p / rho**g
-}
-
 // hll (riemann.f90:47-92), branch-free: both sides' flux and state are
 // formed and the region (0 < sl: left, 0 > sr: right, else the HLL
 // average) is selected, so that the solver is one basic block and
