@@ -228,8 +228,10 @@ int validate(const evh_config *c) {
   if (c->conduction && c->precision == EVH_FP32_MIXED && !fused_supported(*c)) return EVH_ERR_CONFIG;
   // a run without hydro (main.f90:119,155) has only conduction to do
   if (c->hydro_off && !c->conduction) return EVH_ERR_CONFIG;
-  // gravity: 3-D grids (the FFT Poisson solve of gravitation.f90 on one rank)
-  if (c->gravity && (!fused_supported(*c) || c->nz < 2 || !(c->h2 > 0.0) || c->nyqx < 1 || c->nyqy < 1 || c->nyqz < 1 ||
+  // gravity: the FFT Poisson solve of gravitation.f90 on one rank, on 3-D,
+  // 2-D (nz = 1) and 1-D (ny = nz = 1) grids, the reference's reduced
+  // dimensions (parameter.f90:173-182); x always has two cells or more
+  if (c->gravity && (c->nx < 2 || !(c->h2 > 0.0) || c->nyqx < 1 || c->nyqy < 1 || c->nyqz < 1 ||
                      (c->supercomoving && !(c->fb > 0.0))))
     return EVH_ERR_CONFIG;
   return EVH_OK;

@@ -34,7 +34,11 @@
 // whose inverse transform is the two rows of phi.  No pass of its own forms
 // the source, and the spectrum is read and written once per pass.  With
 // other nx or ny, hipFFT's 2-D x-y transforms (one per z plane) surround the
-// z pass; other nz take hipFFT's 3-D plans and k_green.
+// z pass; other nz take hipFFT's 3-D plans and k_green.  2-D (nz = 1) and
+// 1-D (ny = nz = 1) grids take hipFFT's 2-D and 1-D plans and k_green: the
+// reference's 3-D transform is the identity along a one-cell axis, whose
+// only wavenumber is 0 (sin^2 0 in the Green's function), and its force
+// component is 0 (gravitation.f90:209,220,232).
 //
 // Force and update.  k_gravint marches z planes of 64 x 4 column tiles with
 // the potential of the tile and its one-cell ring in LDS (periodic wrap, as
@@ -836,10 +840,22 @@ hipError_t gravity_init(evh_ctx *h) {
     h->dev_bytes += (long long)(sizeof(double2) * tw.size());
     if ((e = hipMemcpy(h->twiddle, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) != hipSuccess)
       return e;
-  } else {
+  } else if (c.nz > 1) {
     if ((e = fft_status(hipfftPlan3d(&pf, c.nz, c.ny, c.nx, HIPFFT_D2Z))) != hipSuccess) return e;
     h->plan_fwd = pf;
     if ((e = fft_status(hipfftPlan3d(&pi, c.nz, c.ny, c.nx, HIPFFT_Z2D))) != hipSuccess) return e;
+    h->plan_inv = pi;
+  } else {
+    // 2-D and 1-D grids: the transform over the axes longer than one cell
+    // (a length-1 axis of the reference's 3-D transform is the identity);
+    // the half spectrum keeps its [nz][ny][nx/2+1] layout
+    if (c.ny > 1) e = fft_status(hipfftPlan2d(&pf, c.ny, c.nx, HIPFFT_D2Z));
+    else e = fft_status(hipfftPlan1d(&pf, c.nx, HIPFFT_D2Z, 1));
+    if (e != hipSuccess) return e;
+    h->plan_fwd = pf;
+    if (c.ny > 1) e = fft_status(hipfftPlan2d(&pi, c.ny, c.nx, HIPFFT_Z2D));
+    else e = fft_status(hipfftPlan1d(&pi, c.nx, HIPFFT_Z2D, 1));
+    if (e != hipSuccess) return e;
     h->plan_inv = pi;
   }
   if (pf && (e = fft_status(hipfftSetStream(pf, h->stream))) != hipSuccess) return e;

@@ -33,8 +33,8 @@
 ! filled on the host; pure hydro never reads them there.  Thermal
 ! conduction (thermalstep, thermalconduction with simplekappa, without dual
 ! energy; on 1-, 2- and 3-D grids, also with hydro = .false., when main.f90
-! calls neither hydrostep nor sweep) and self-gravity on 3-D grids
-! (gravpotential, gravintegration; the potential stays on the device) run
+! calls neither hydrostep nor sweep) and self-gravity on 1-, 2- and 3-D
+! grids (gravpotential, gravintegration; the potential stays on the device) run
 ! on the device too.  Physics that
 ! update q on the host between steps (chemistry, floors), supercomoving
 ! runs (whose host-side time and snapshot rules this dispatch does not
@@ -136,10 +136,6 @@ contains
     if (chemistry .or. supercomoving .or. tempfloor .or. jeansfloor &
          .or. cooldt .or. noncie .or. parallel .or. (.not. hydro .and. .not. conduction)) then
        print *, 'ERROR: EVORA_BACKEND=hip runs hydro (with thermal conduction and gravity) on one rank only'
-       call cleanstop
-    endif
-    if (gravity .and. (onedimension .or. twodimension)) then
-       print *, 'ERROR: EVORA_BACKEND=hip runs gravity on 3-D grids only'
        call cleanstop
     endif
     if (conduction .and. (.not. simplekappa .or. dualenergy)) then

@@ -59,8 +59,9 @@ typedef struct evh_config {
    * simplekappa, without dual energy; 1-D and 2-D grids in EVH_MODE_STAGED */
   int conduction, simplekappa;
   double c_cond, temp0, kappa0;
-  /* self-gravity (parameter.f90:235-241,254-263): 3-D, one rank; the
-   * potential is periodic whatever the boundtype, as in the reference */
+  /* self-gravity (parameter.f90:235-241,254-263): 3-D, 2-D (nz = 1) and
+   * 1-D (ny = nz = 1) grids with nx >= 2, one rank; the potential is
+   * periodic whatever the boundtype, as in the reference */
   int gravity, supercomoving;
   double grav, fb, h2, kfx2, kfy2, kfz2;
   int nyqx, nyqy, nyqz;

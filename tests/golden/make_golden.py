@@ -127,12 +127,13 @@ def main():
 def gravity():
     """multigreen and gravintegration (gravitation.f90:55-89,172-254) of the
     reference on seeded fields, for grids with even, odd and power-of-two
-    sizes (grid spacings differ per axis).  The reference's full potential
-    needs FFTW-2, which this image lacks; these two routines do not."""
+    sizes (grid spacings differ per axis), and 2-D (nz = 1) and 1-D
+    (ny = nz = 1) grids.  The reference's full potential needs FFTW-2, which
+    this image lacks; these two routines do not."""
     fx = {}
     with open(os.path.join(HERE, "index.json")) as fh:
         index = json.load(fh)
-    for nx, ny, nz in ((8, 6, 4), (16, 16, 16), (5, 7, 3), (12, 10, 9)):
+    for nx, ny, nz in ((8, 6, 4), (16, 16, 16), (5, 7, 3), (12, 10, 9), (9, 12, 1), (16, 8, 1), (16, 1, 1), (7, 1, 1)):
         p = HydroParams(nx=nx, ny=ny, nz=nz, gravity=True, atomic=True, xmax=2.0, ymax=1.5)
         rng = np.random.default_rng(1000 * nx + 10 * ny + nz)
         a = rng.standard_normal((nz, ny, nx)) + 1j * rng.standard_normal((nz, ny, nx))

@@ -216,8 +216,10 @@ def test_gravity_dropin(exact):
 
 
 def test_gravity_refusals():
+    """A one-cell x axis is no reduced dimension of the reference (2-D means
+    nz = 1, 1-D ny = nz = 1, parameter.f90:173-182) and is refused."""
     with pytest.raises(hydro.EvoraError) as ei:
-        hydro.Hydro(HydroParams(nx=16, ny=16, nz=1, gravity=True).derive())
+        hydro.Hydro(HydroParams(nx=1, ny=16, nz=16, gravity=True).derive())
     assert ei.value.code == 8
     p = ics.jeans_params(8)
     with hydro.Hydro(p.derive()) as h:
@@ -262,3 +264,77 @@ def test_gravity_smallest_grids(shape):
         q = h.download_q()
     for c in range(6):
         assert rel_linf(q[..., c], qo[..., c]) <= 1e-12, c
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 1), (12, 10, 1), (9, 12, 1), (64, 32, 1), (2, 2, 1), (16, 1, 1),
+                                   (7, 1, 1), (2, 1, 1)])
+def test_potential_reduced_dimensions(shape):
+    """2-D (nz = 1) and 1-D (ny = nz = 1) grids take hipFFT's 2-D and 1-D
+    plans; the reference's 3-D transform is the identity along a one-cell
+    axis and the Green's function has that axis's wavenumber 0.  A random
+    density puts signal in every bin."""
+    nx, ny, nz = shape
+    p = ics.jeans_params(nx, ny=ny, nz=nz, xmax=nx / 16, ymax=ny / 16, zmax=nz / 16)
+    d = p.derive()
+    q = ics.jeans(p)
+    rng = np.random.default_rng(nx * ny + nz)
+    q[..., 0] = 1e6 * (1.0 + 0.1 * rng.standard_normal(q.shape[:3]))
+    phi = _potential(d, q)
+    ref = O.Oracle(d).gravpotential(q)
+    assert rel_linf(phi, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("shape,mode,tol", [((24, 20, 1), hydro.MODE_FUSED, 1e-10),
+                                            ((24, 20, 1), hydro.MODE_FUSED_EXACT, 1e-12),
+                                            ((24, 20, 1), hydro.MODE_STAGED, 1e-12),
+                                            ((32, 1, 1), hydro.MODE_STAGED, 1e-12)])
+def test_gravity_reduced_dimensions_steps(shape, mode, tol):
+    """5 steps of a 2-D (the single-pass 2-D sweep, fast and exact, and the
+    per-routine kernels) and a 1-D Jeans box (the per-routine kernels) with
+    gravity against the oracle; the force has no z (and in 1-D no y)
+    component (gravitation.f90:209,220,232)."""
+    nx, ny, nz = shape
+    p = ics.jeans_params(nx, ny=ny, nz=nz, xmax=nx / 24, ymax=ny / 24)
+    d = p.derive()
+    q0 = ics.jeans(p)
+    qo = q0.copy()
+    to, _, _ = O.Oracle(d).run(qo, 5)
+    with hydro.Hydro(d, mode=mode) as h:
+        h.upload_q(q0)
+        t, _, n = h.run(5)
+        q = h.download_q()
+    assert n == 5
+    assert t == pytest.approx(to, rel=1e-12)
+    for c in range(6):
+        assert rel_l1(q[..., c], qo[..., c]) <= tol, c
+
+
+def test_gravity_2d_mixed_precision():
+    p = ics.jeans_params(32, nz=1)
+    d = p.derive()
+    q0 = ics.jeans(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 5)
+    with hydro.Hydro(d, precision=hydro.FP32_MIXED) as h:
+        h.upload_q(q0)
+        h.run(5)
+        q = h.download_q()
+    for c in (0, 4):
+        assert rel_l1(q[..., c], qo[..., c]) <= 1e-5, c
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_gravity_dropin_2d(exact):
+    """The Fortran drop-in with gravity on a 2-D grid (twodimension in the
+    reference's driver) against the oracle's run."""
+    p = ics.jeans_params(24, nz=1, tend=1e30)
+    q0 = ics.jeans(p)
+    env = {"EVORA_MAXSTEPS": "6"}
+    if exact:
+        env["EVORA_EXACT"] = "1"
+    qh, sh = run_dropin(p, q0, 1000, "hip", env)
+    assert len(sh) == 7
+    qo = q0.copy()
+    O.Oracle(p.derive()).run(qo, 6)
+    for c in range(6):
+        assert rel_l1(qh[..., c], qo[..., c]) <= 1e-10, c
