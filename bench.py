@@ -23,7 +23,9 @@ with self-gravity on (against the hydro step alone, with the potential and
 the force kernel timed, on a 512^3 Jeans-unstable box), and the reference's
 CPU hot path timed on the host's cores on a bounded sample.
 The single-pass 2-D form is timed on the 8192^2 shear layer of
-para/kevinhelmholtz (forms.two_d).  Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
+para/kevinhelmholtz (forms.two_d), and one GPU's share of the blast split
+into 8 z-slabs, a 512 x 512 x 64 slab with its ghost planes exchanged
+through halo buffers every step, on its own (forms.zslab).  Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
 HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50 steps, the
 cosmological-like field 256^3 for 100 steps, the Jeans box with gravity and
 the hot blob with conduction at 256^3 for 50 steps, and the batched
@@ -212,6 +214,75 @@ def run_gravity(n, steps, warmup, fp64_only=False):
                          gravintegration_GBs=ach, gravintegration_hbm_frac=ach / HBM_PEAK_GBS)
         log(f"gravity {prec}: potential {r['gravpot_ms']:.3f} ms, force {r['gravint_ms']:.3f} ms ({ach:.0f} GB/s), "
             f"step {r['ms_per_step']:.3f} ms vs hydro alone {rh['ms_per_step']:.3f} ms")
+    return out
+
+
+def run_zslab(n, nranks, steps, warmup, whole_kernel_ms):
+    """One GPU's share of an n^3 grid split into nranks z-slabs
+    (parallel.f90:53-61): an n x n x n/nranks slab whose two ghost planes on
+    each side come from halo buffers (evh_slab) filled every step by the
+    exchange communicate makes (parallel.f90:317-381), here the slab's own
+    boundary planes copied into its halos on the device (a z-periodic slab)
+    in place of the transfer from the neighbours.  FP64 fused, HLLC, the
+    blast slab through the blast centre.  For each z chunk the sweep kernel
+    is timed per launch; the step (exchange copy, hydrostep, sweep) by wall
+    clock.  Its efficiency is the slab kernel's time per cell against the
+    whole grid's."""
+    from evora_amd import hydro, ics
+    from evora_amd.params import zslab_ranges
+
+    a, b = zslab_ranges(n, nranks)[nranks // 2]
+    nzs = b - a
+    p = ics.blast_params(n, nz=nzs, zmin=0.0, zmax=nzs / n, solver=2, cfl=0.4)
+    d = p.derive()
+    q0 = ics.blast(p)
+    cells = d.ncells
+    out = {"workload": f"z-slab {n}x{n}x{nzs} (rank {nranks // 2} of {nranks} of the {n}^3 blast, planes {a}-{b - 1}), "
+                       "both ghost-plane pairs from halo buffers, HLLC, minmod, CFL 0.4, FP64 fused",
+           "cells": cells, "by_zchunk": {}}
+    best = None
+    for zc in (64, 32, 16):
+        with hydro.Hydro(d) as h:
+            h.set_tuning(12, True, zc)
+            h.upload_q(q0)
+            lo, hi = h.slab(True, True)
+
+            def step(t):
+                h.synchronize()
+                h.slab_faces(hi, lo)  # the planes the neighbours would send, into this slab's halos
+                h.primitive()
+                dt = h.hydrostep(1e99)
+                h.compevo(dt)
+                h.sweep(dt)
+                return t + dt
+
+            t = 0.0
+            for _ in range(warmup):
+                t = step(t)
+            h.synchronize()
+            h.set_timing(True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                t = step(t)
+            h.synchronize()
+            wall = time.perf_counter() - t0
+            kms, nl = h.kernel_stats()
+            t1 = time.perf_counter()
+            for _ in range(20):
+                h.slab_faces(hi, lo)
+            faces_ms = 1e3 * (time.perf_counter() - t1) / 20
+        kernel_ms = kms / max(nl, 1)
+        r = dict(ms_per_step=1e3 * wall / steps, value=cells * steps / wall, kernel_ms=kernel_ms,
+                 exchange_copy_ms=faces_ms,
+                 kernel_efficiency_vs_whole_grid=(whole_kernel_ms / n ** 3) / (kernel_ms / cells))
+        out["by_zchunk"][zc] = r
+        log(f"z-slab zchunk {zc}: kernel {kernel_ms:.3f} ms, step {r['ms_per_step']:.3f} ms, "
+            f"exchange copy {faces_ms:.3f} ms, per-cell efficiency {r['kernel_efficiency_vs_whole_grid']:.3f}")
+        if best is None or kernel_ms < out["by_zchunk"][best]["kernel_ms"]:
+            best = zc
+    out["best_zchunk"] = best
+    out.update({k: out["by_zchunk"][best][k] for k in ("value", "ms_per_step", "kernel_ms",
+                                                       "kernel_efficiency_vs_whole_grid")})
     return out
 
 
@@ -504,6 +575,8 @@ def main():
     ap.add_argument("--conduction-only", action="store_true",
                     help="run only the thermal-conduction form and print its JSON (for profiling)")
     ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity form")
+    ap.add_argument("--no-zslab", action="store_true", help="skip one GPU's z-slab of an 8-way split")
+    ap.add_argument("--zslab-only", action="store_true", help="run only the z-slab form")
     ap.add_argument("--gravity-only", action="store_true",
                     help="run only the self-gravity form and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
@@ -531,6 +604,11 @@ def main():
         return
     if args.gravity_only:
         print(json.dumps(run_gravity(args.n, args.steps, args.warmup, args.fp64_only)), flush=True)
+        return
+    if args.zslab_only:
+        pw = ics.blast_params(args.n, solver=2, cfl=args.cfl)
+        whole = run_gpu(pw.derive(), ics.blast(pw), args.steps, args.warmup)
+        print(json.dumps(run_zslab(args.n, 8, args.steps, args.warmup, whole["kernel_ms"])), flush=True)
         return
     p = ics.blast_params(args.n, solver=2, cfl=args.cfl)
     d = p.derive()
@@ -582,6 +660,8 @@ def main():
         forms["conduction"] = run_conduction(args.n, args.steps, args.warmup, args.fp64_only)
     if not args.no_gravity:
         forms["gravity"] = run_gravity(args.n, args.steps, args.warmup, args.fp64_only)
+    if not args.no_zslab:
+        forms["zslab"] = run_zslab(args.n, 8, args.steps, args.warmup, fp64["kernel_ms"])
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
     if os.path.exists(tpath):

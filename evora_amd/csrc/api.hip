@@ -338,6 +338,7 @@ int evh_destroy(evh_ctx *h) {
   if (h->st) hipFree(h->st);
   if (h->st_host) hipHostFree(h->st_host);
   if (h->partials) hipFree(h->partials);
+  for (auto p : h->zhalo) if (p) hipFree(p);
   for (auto ev : h->events) if (ev) hipEventDestroy(ev);
   for (auto &pair : h->tev)
     for (auto ev : pair)
@@ -353,6 +354,8 @@ int evh_set_mode(evh_ctx *h, int mode) {
   if (mode != EVH_MODE_STAGED && !fused_supported(h->cfg))
     return fail(h, EVH_ERR_CONFIG, "fused needs a 2-D or 3-D grid");
   if (mode != EVH_MODE_FUSED && mixed(h)) return fail(h, EVH_ERR_CONFIG, "staged and exact kernels are FP64");
+  if (mode == EVH_MODE_STAGED && (h->zslab[0] || h->zslab[1]))
+    return fail(h, EVH_ERR_CONFIG, "a z-slab runs the fused kernels only");
   h->mode = mode;
   h->v_valid = h->tv_valid = h->t_valid = false;  // limits and T are formed in the mode's arithmetic
   h->w_valid = false;
@@ -574,6 +577,8 @@ int evh_download_phi(evh_ctx *h, double *phi) {
 
 int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go) {
   if (!h || !t || !dt || !go) return EVH_ERR_STATE;
+  if (h->zslab[0] || h->zslab[1])
+    return fail(h, EVH_ERR_STATE, "a z-slab steps through hydrostep and sweep, with the exchange between");
   int r;
   if ((r = evh_primitive(h))) return r;
   if ((r = evh_bound(h))) return r;
@@ -759,6 +764,53 @@ int evh_stage_stats(evh_ctx *h, int stage, double *total_ms, long long *launches
   if (r) return r;
   *total_ms = h->stage_ms[stage];
   *launches = h->stage_launches[stage];
+  return EVH_OK;
+}
+
+// ---------------------------------------------------------------- z-slabs
+// One rank's share of a grid decomposed into z-slabs (parallel.f90:26-66):
+// the ghost planes on a side with a neighbour are the neighbour's planes
+// (communicate, parallel.f90:317-381), which the caller places in the halo
+// buffer of that side; a side without one keeps the z rules of bound
+// (boundary.f90:90-110).  q, not w, crosses: the march forms the primitive
+// of every plane it reads.
+int evh_slab(evh_ctx *h, int lo, int hi, void **lo_halo, void **hi_halo) {
+  if (!h) return EVH_ERR_STATE;
+  if (h->mode == EVH_MODE_STAGED) return fail(h, EVH_ERR_CONFIG, "a z-slab runs the fused kernels only");
+  if (h->grid.nz < 2) return fail(h, EVH_ERR_CONFIG, "a z-slab has two planes or more");
+  if (h->cfg.conduction || h->cfg.gravity || h->cfg.hydro_off)
+    return fail(h, EVH_ERR_CONFIG, "a z-slab runs the hydro step only");
+  const bool on[2] = {lo != 0, hi != 0};
+  const size_t bytes = h->qelem * 12 * (size_t)h->grid.nx * h->grid.ny;
+  for (int side = 0; side < 2; side++) {
+    if (on[side] && !h->zhalo[side]) {
+      HIP_TRY(hipMalloc(&h->zhalo[side], bytes));
+      h->dev_bytes += (long long)bytes;
+      HIP_TRY(hipMemsetAsync(h->zhalo[side], 0, bytes, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    h->zslab[side] = on[side];
+  }
+  if (lo_halo) *lo_halo = on[0] ? h->zhalo[0] : nullptr;
+  if (hi_halo) *hi_halo = on[1] ? h->zhalo[1] : nullptr;
+  return EVH_OK;
+}
+
+// The two lowest and two highest planes of the current q, [6][2][ny][nx],
+// into device memory: the planes communicate sends to the neighbours below
+// and above.
+int evh_slab_faces(evh_ctx *h, void *lo_dst, void *hi_dst) {
+  if (!h) return EVH_ERR_STATE;
+  if (h->grid.nz < 2) return fail(h, EVH_ERR_CONFIG, "a z-slab has two planes or more");
+  const size_t pl = (size_t)h->grid.nx * h->grid.ny, e = h->qelem;
+  const size_t n = (size_t)h->grid.n();
+  const char *q = (const char *)h->qbuf[h->cur];
+  if (lo_dst)
+    HIP_TRY(hipMemcpy2DAsync(lo_dst, 2 * pl * e, q, n * e, 2 * pl * e, 6, hipMemcpyDeviceToDevice, h->stream));
+  if (hi_dst)
+    HIP_TRY(hipMemcpy2DAsync(hi_dst, 2 * pl * e, q + (h->grid.nz - 2) * pl * e, n * e, 2 * pl * e, 6,
+                             hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return EVH_OK;
 }
 

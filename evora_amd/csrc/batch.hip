@@ -332,6 +332,7 @@ int describe_boxes(const evh_config *cfgs, const KParams *kps, const long long *
     g.tiles_y = (g.ny + ty - 1) / ty;
     g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
     g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
+    g.zlo = g.zhi = nullptr;  // every box is a whole grid
     out[i].P = kps[i];
     out[i].G = g;
     out[i].qoff = 6 * qoff[i];

@@ -71,6 +71,12 @@ struct evh_ctx {
   bool phi_valid = false;
   double cosmo = 1.0;  // expansion factor of the last evh_gravpotential
 
+  // a z-slab of a grid decomposed along z (evh_slab): the ghost planes below
+  // ([0]) and above ([1]) from the neighbours' planes in zhalo, [6][2][ny][nx]
+  // in the storage type, instead of the z rules of bound
+  bool zslab[2] = {false, false};
+  void *zhalo[2] = {nullptr, nullptr};
+
   // staged path arrays (allocated on first use)
   double *w = nullptr, *evo = nullptr, *s = nullptr;
 

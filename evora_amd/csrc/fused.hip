@@ -37,14 +37,16 @@ using core::Geo;
 
 // WPE: waves per SIMD the register allocation must leave room for
 // (512 / WPE VGPRs per lane); the tuning knob next to the tile height TY.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
+// SLAB: a z-slab whose ghost planes may come from halo buffers.
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE, bool SLAB>
 __global__ void __launch_bounds__(core::TileShape<TY>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     K_FUSED(KParams P, Geo G, const S *__restrict__ qin, S *__restrict__ qout, double dt, Status *st,
             S *__restrict__ tout) {
   __shared__ core::Smem<T, TY> sm;
   const int lin = xcd_tile(blockIdx.x, G.ntiles);
   if (lin < 0) return;  // padding block (whole block exits before any barrier)
-  core::march<T, S, SOLVER, LIMITER, TY, kExact>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm, tout);
+  core::march<T, S, SOLVER, LIMITER, TY, kExact, SLAB>(P, G, qin, qout, 0, 0, dt, st, &st->vmin_bits, lin, sm,
+                                                        tout);
 }
 
 // 2-D grids (nz = 1): row strips marching along y (march2d.h)
@@ -87,14 +89,16 @@ Geo make_geo(const evh_ctx *h, int ty) {
   g.tiles_y = (g.ny + ty - 1) / ty;
   g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
   g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
+  g.zlo = h->zslab[0] ? h->zhalo[0] : nullptr;
+  g.zhi = h->zslab[1] ? h->zhalo[1] : nullptr;
   return g;
 }
 
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE>
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, int WPE, bool SLAB = false>
 void launch_ty(evh_ctx *h, double dt) {
   const Geo g = make_geo(h, TY);
   const int blocks = xcd_blocks(g.ntiles);
-  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
+  hipLaunchKernelGGL((K_FUSED<T, S, SOLVER, LIMITER, TY, WPE, SLAB>), dim3(blocks), dim3(core::TileShape<TY>::NT), 0,
                      h->stream, h->kp, g, (const S *)h->qbuf[h->cur], (S *)h->qbuf[1 - h->cur], dt, h->st,
                      (S *)h->tbuf);
 }
@@ -127,6 +131,12 @@ void launch_fused_t(evh_ctx *h, double dt) {
     // wave per SIMD spills 64 / 32 B per lane and measured 10% / 27% slower
     // at 8192^2)
     launch_2d<T, S, SOLVER, LIMITER, sizeof(T) == 8 ? 2 : 4>(h, dt);
+    return;
+  }
+  if (h->zslab[0] || h->zslab[1]) {
+    // a z-slab with halo buffers: the default tile shape (the z chunk is
+    // still the tuning's)
+    launch_ty<T, S, SOLVER, LIMITER, 12, HI, true>(h, dt);
     return;
   }
   const bool hi = h->wpe_hi;

@@ -58,6 +58,10 @@ struct Geo {
   int zchunk;
   int tiles_x, tiles_y, nchunks;
   int ntiles;  // tiles_x * tiles_y * nchunks
+  // a z-slab of a grid decomposed along z: the two ghost planes below (zlo)
+  // and above (zhi) come from the neighbouring slabs, [6][2][ny][nx] in the
+  // storage type; null: the z rules of bound
+  const void *zlo, *zhi;
 };
 
 template <int TY>
@@ -320,11 +324,34 @@ __device__ __forceinline__ void fetch_at(const Geo &G, const S *__restrict__ q, 
   for (int m = 0; m < 6; m++) raw[m] = q[m * G.n + c];
 }
 
+// raw q of column offset c in plane gz (clamped as plane_offset does).
+// Inside the grid, and on a side without a neighbour, that is q with the
+// ghost rules of bound; in a z-slab (SLAB) the ghost planes beyond the slab
+// are the neighbour's planes in the halo buffer of that side, [6][2][ny][nx]
+// (the planes communicate receives, parallel.f90:317-381).  Whole grids take
+// the kernels without the test.
+template <bool SLAB, typename S>
+__device__ __forceinline__ void fetch_plane(const KParams &P, const Geo &G, const S *__restrict__ q, int gz,
+                                            long long c, S raw[6]) {
+  if constexpr (SLAB) {
+    const long long pl = (long long)G.nx * G.ny;
+    const S *h = nullptr;
+    if (gz < 0 && G.zlo) h = (const S *)G.zlo + pl * (gz < -2 ? 0 : gz + 2);
+    else if (gz >= G.nz && G.zhi) h = (const S *)G.zhi + pl * (gz > G.nz ? 1 : 0);
+    if (h) {
+#pragma unroll
+      for (int m = 0; m < 6; m++) raw[m] = h[m * 2 * pl + c];
+      return;
+    }
+  }
+  fetch_at(G, q, c + plane_offset(P, G, gz), raw);
+}
+
 // raw q at (gx, gy, gz), ghost-mapped
-template <typename S>
+template <bool SLAB, typename S>
 __device__ __forceinline__ void fetch_raw(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
                                           int gz, S raw[6]) {
-  fetch_at(G, q, col_offset(P, G, gx, gy) + plane_offset(P, G, gz), raw);
+  fetch_plane<SLAB>(P, G, q, gz, col_offset(P, G, gx, gy), raw);
 }
 
 // primitive of a fetched cell at column gx (for the reflecting wall)
@@ -342,11 +369,11 @@ __device__ __forceinline__ void prim_cell(const KParams &P, const Geo &G, const 
 }
 
 // load q at (gx, gy, gz) (ghost-mapped) and recover w
-template <bool EXACT, typename S, typename T>
+template <bool EXACT, bool SLAB, typename S, typename T>
 __device__ __forceinline__ void load_cell(const KParams &P, const Geo &G, const S *__restrict__ q, int gx, int gy,
                                           int gz, T qc[6], T wc[5], T &rinv) {
   S raw[6];
-  fetch_raw(P, G, q, gx, gy, gz, raw);
+  fetch_raw<SLAB>(P, G, q, gx, gy, gz, raw);
   prim_cell<EXACT>(P, G, raw, gx, qc, wc, rinv);
 }
 
@@ -369,8 +396,9 @@ struct Smem {
 // at element qoff of qin_all/qout_all.  vmin_dst receives the CFL minimum of
 // the tile's new state; errors carry the box index above bit 40 of the cell.
 // With tout (single box only), the temperature of each new cell is stored
-// there too, for the conduction pass that follows the sweep.
-template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT>
+// there too, for the conduction pass that follows the sweep.  SLAB: the grid
+// is a z-slab whose ghost planes may come from halo buffers (Geo::zlo, zhi).
+template <typename T, typename S, int SOLVER, int LIMITER, int TY, bool EXACT, bool SLAB = false>
 __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__restrict__ qin_all,
                                       S *__restrict__ qout_all, long long qoff, int box, double dt, Status *st,
                                       unsigned long long *vmin_dst, int lin, Smem<T, TY> &sm,
@@ -454,14 +482,14 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
   T rw2[5];
 #pragma unroll
   for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); }
-  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
-  load_cell<EXACT, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
-  fetch_at(G, qin, own_xy + plane_offset(P, G, z0), pre);
+  load_cell<EXACT, SLAB, S, T>(P, G, qin, gx, gy, z0 - 2, tq, wm, rm);
+  load_cell<EXACT, SLAB, S, T>(P, G, qin, gx, gy, z0 - 1, tq, wc, rc);
+  fetch_plane<SLAB>(P, G, qin, z0, own_xy, pre);
   {
     T rr2;
-    fetch_at(G, qin, r2_xy + plane_offset(P, G, z0 - 1), pre2);
+    fetch_plane<SLAB>(P, G, qin, z0 - 1, r2_xy, pre2);
     prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
-    fetch_at(G, qin, r2_xy + plane_offset(P, G, z0), pre2);
+    fetch_plane<SLAB>(P, G, qin, z0, r2_xy, pre2);
   }
 
   for (int k = z0 - 1; k <= z1; k++) {
@@ -471,7 +499,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     // clamped coordinates), so that no branch makes a load wait for the
     // fetches issued after it.
     prim_cell<EXACT>(P, G, pre, gx, qn, wp, rp);
-    fetch_at(G, qin, own_xy + plane_offset(P, G, k + 2), pre);
+    fetch_plane<SLAB>(P, G, qin, k + 2, own_xy, pre);
     if (ring && r2x > -3) store_w<T, EA>(W, eidx<TY>(r2x, r2y), rw2);
     if (owns_col) store_w<T, EA>(W, me, wc);
     __syncthreads();
@@ -554,7 +582,7 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
     if (ring) {  // second-ring w of plane k+1, stored in S1 of plane k+1
       T rr2;
       prim_cell<EXACT>(P, G, pre2, x0 + r2x, tq, rw2, rr2);
-      fetch_at(G, qin, r2_xy + plane_offset(P, G, k + 2), pre2);
+      fetch_plane<SLAB>(P, G, qin, k + 2, r2_xy, pre2);
     }
     __syncthreads();
 

@@ -70,6 +70,21 @@ module evora_hip
        real(c_double), value :: dt
      end function evh_gravintegration
 
+     ! one rank's z-slab (parallel.f90:26-66): halo buffers for the ghost
+     ! planes communicate (parallel.f90:317-381) receives, and the boundary
+     ! planes it sends, as device addresses
+     integer(c_int) function evh_slab(h, lo, hi, lo_halo, hi_halo) bind(c, name='evh_slab')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: lo, hi
+       type(c_ptr) :: lo_halo, hi_halo
+     end function evh_slab
+
+     integer(c_int) function evh_slab_faces(h, lo_dst, hi_dst) bind(c, name='evh_slab_faces')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, lo_dst, hi_dst
+     end function evh_slab_faces
+
      integer(c_int) function evh_destroy(h) bind(c, name='evh_destroy')
        import :: c_int, c_ptr
        type(c_ptr), value :: h

@@ -40,6 +40,7 @@ EXPORTS = (
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
     "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_conduction_stats", "evh_stage_stats",
     "evh_gravpotential", "evh_gravintegration", "evh_download_phi",
+    "evh_slab", "evh_slab_faces",
     "evh_device_bytes", "evh_math_eval",
     "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
     "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
@@ -114,6 +115,8 @@ def load_library():
             "evh_gravpotential": [P, C.c_double],
             "evh_gravintegration": [P, C.c_double],
             "evh_download_phi": [P, D],
+            "evh_slab": [P, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+            "evh_slab_faces": [P, C.c_void_p, C.c_void_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -266,6 +269,22 @@ class Hydro:
         phi = np.empty((self.d.nz, self.d.ny, self.d.nx))
         self._check(self.L.evh_download_phi(self.h, _dptr(phi)))
         return phi
+
+    # --------------------------------------------------------------- z-slabs
+    def slab(self, lo: bool, hi: bool):
+        """This grid as one rank's z-slab (parallel.f90:26-66): the ghost planes
+        below (lo) and above (hi) come from the neighbours' planes, placed in
+        the returned device halo buffers (addresses, None for a side that
+        keeps bound's z rules)."""
+        a, b = C.c_void_p(), C.c_void_p()
+        self._check(self.L.evh_slab(self.h, int(bool(lo)), int(bool(hi)), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def slab_faces(self, lo_dst, hi_dst):
+        """The two lowest and two highest planes of q into device buffers
+        (addresses or None): the planes communicate sends
+        (parallel.f90:317-381)."""
+        self._check(self.L.evh_slab_faces(self.h, lo_dst, hi_dst))
 
     def step(self, t: float, tend: float):
         """One pass of main.f90:109-155; returns (t, dt, go)."""

@@ -184,3 +184,17 @@ class Derived:
     @property
     def ncells(self) -> int:
         return self.nx * self.ny * self.nz
+
+
+def zslab_ranges(nz: int, nranks: int):
+    """The z-slab of each rank, as 0-based [lo, hi) plane ranges: the split of
+    decomposition (parallel.f90:53-61), rank r taking planes lz..mz with
+    mz = lz + (nz - lz) / (nranks - r) after the ranks below it."""
+    if not 1 <= nranks <= nz:
+        raise ValueError("every rank needs a plane")  # parallel.f90:42-49
+    out, mz = [], 0
+    for r in range(nranks):
+        lz = mz + 1
+        mz = lz + (nz - lz) // (nranks - r)
+        out.append((lz - 1, mz))
+    return out

@@ -121,6 +121,30 @@ int evh_download_phi(evh_ctx *h, double *phi);    /* the potential phi(nx,ny,nz)
 int evh_step(evh_ctx *h, double *t, double tend, double *dt, int *go);
 int evh_run(evh_ctx *h, int nsteps, double *t, double tend, double *dt, int *done);
 
+/* ---- one rank's z-slab of a grid decomposed along z (parallel.f90:26-66;
+ * the context's nz is the slab's, every other parameter the whole grid's) ----
+ * evh_slab: with lo (hi) nonzero the two ghost planes below (above) the slab
+ * are read from a halo buffer of the context instead of the z rules of bound
+ * (boundary.f90:90-110): the planes communicate (parallel.f90:317-381)
+ * receives from the neighbouring slab, as q.  Each buffer holds the 6
+ * conserved components of the two planes, [6][2][ny][nx], in the storage
+ * type (double; float with EVH_FP32_MIXED); *lo_halo / *hi_halo receive
+ * their device addresses (NULL for a side that keeps bound's rules; either
+ * pointer may be NULL).  The caller fills them before each evh_sweep, while
+ * the context is idle (its sweep kernel reads them: after evh_synchronize or
+ * evh_hydrostep, which waits for the context's stream), and
+ * drives the step through evh_primitive, evh_hydrostep (dt reduced to the
+ * minimum over the slabs, reducestep parallel.f90:155-165), evh_compevo and
+ * evh_sweep; evh_step and evh_run refuse a slab.  Fused modes on grids of
+ * two planes or more, hydro only (no conduction or gravity). */
+int evh_slab(evh_ctx *h, int lo, int hi, void **lo_halo, void **hi_halo);
+/* The slab's two lowest (lo_dst) and two highest (hi_dst) planes of the
+ * current q, [6][2][ny][nx] in the storage type, into device memory (a
+ * neighbour's halo buffer, or a buffer a send reads): the planes
+ * communicate sends (parallel.f90:334,348).  Either pointer may be NULL;
+ * complete when it returns. */
+int evh_slab_faces(evh_ctx *h, void *lo_dst, void *hi_dst);
+
 /* snapshot.f90:169-172 sums of q1, q5, q6 (without the volume factor), FP64
  * compensated on the device. */
 int evh_sums(evh_ctx *h, double out[3]);
