@@ -164,7 +164,9 @@ int evh_synchronize(evh_ctx *h);
 /* Launch shape of the fused kernel (no counterpart in the reference; set by
  * the autotuner): tile height 8 or 12 columns, occupancy target (0: the
  * lower, 1: the higher of the two register budgets compiled per precision),
- * z planes per block (0: default 128).  Invalid values are rejected. */
+ * z planes per block (0: default 128).  Invalid values are rejected.  A
+ * z-slab with halo buffers (evh_slab) runs tile height 12 at the higher
+ * budget; its z chunk is the tuning's. */
 int evh_set_tuning(evh_ctx *h, int tile_ty, int wpe_hi, int zchunk);
 int evh_get_tuning(evh_ctx *h, int *tile_ty, int *wpe_hi, int *zchunk);
 
