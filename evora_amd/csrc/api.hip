@@ -339,6 +339,7 @@ int evh_destroy(evh_ctx *h) {
   if (h->st_host) hipHostFree(h->st_host);
   if (h->partials) hipFree(h->partials);
   for (auto p : h->zhalo) if (p) hipFree(p);
+  for (auto p : h->thalo) if (p) hipFree(p);
   for (auto ev : h->events) if (ev) hipEventDestroy(ev);
   for (auto &pair : h->tev)
     for (auto ev : pair)
@@ -778,19 +779,23 @@ int evh_slab(evh_ctx *h, int lo, int hi, void **lo_halo, void **hi_halo) {
   if (!h) return EVH_ERR_STATE;
   if (h->mode == EVH_MODE_STAGED) return fail(h, EVH_ERR_CONFIG, "a z-slab runs the fused kernels only");
   if (h->grid.nz < 2) return fail(h, EVH_ERR_CONFIG, "a z-slab has two planes or more");
-  if (h->cfg.conduction || h->cfg.gravity || h->cfg.hydro_off)
-    return fail(h, EVH_ERR_CONFIG, "a z-slab runs the hydro step only");
+  if (h->cfg.gravity) return fail(h, EVH_ERR_CONFIG, "a z-slab has no distributed Poisson solve");
   const bool on[2] = {lo != 0, hi != 0};
-  const size_t bytes = h->qelem * 12 * (size_t)h->grid.nx * h->grid.ny;
+  const size_t pl = (size_t)h->grid.nx * h->grid.ny;
   for (int side = 0; side < 2; side++) {
     if (on[side] && !h->zhalo[side]) {
-      HIP_TRY(hipMalloc(&h->zhalo[side], bytes));
-      h->dev_bytes += (long long)bytes;
-      HIP_TRY(hipMemsetAsync(h->zhalo[side], 0, bytes, h->stream));
-      HIP_TRY(hipStreamSynchronize(h->stream));
+      HIP_TRY(hipMalloc(&h->zhalo[side], 12 * pl * h->qelem));
+      h->dev_bytes += (long long)(12 * pl * h->qelem);
+      HIP_TRY(hipMemsetAsync(h->zhalo[side], 0, 12 * pl * h->qelem, h->stream));
+    }
+    if (on[side] && h->cfg.conduction && !h->thalo[side]) {
+      HIP_TRY(hipMalloc(&h->thalo[side], pl * h->qelem));
+      h->dev_bytes += (long long)(pl * h->qelem);
+      HIP_TRY(hipMemsetAsync(h->thalo[side], 0, pl * h->qelem, h->stream));
     }
     h->zslab[side] = on[side];
   }
+  HIP_TRY(hipStreamSynchronize(h->stream));
   if (lo_halo) *lo_halo = on[0] ? h->zhalo[0] : nullptr;
   if (hi_halo) *hi_halo = on[1] ? h->zhalo[1] : nullptr;
   return EVH_OK;
@@ -810,6 +815,35 @@ int evh_slab_faces(evh_ctx *h, void *lo_dst, void *hi_dst) {
   if (hi_dst)
     HIP_TRY(hipMemcpy2DAsync(hi_dst, 2 * pl * e, q + (h->grid.nz - 2) * pl * e, n * e, 2 * pl * e, 6,
                              hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return EVH_OK;
+}
+
+// With conduction on: the halo buffers of T for the plane next to the slab
+// on each side with a neighbour ([ny][nx], the storage type), filled before
+// each evh_thermalconduction (main.f90:170-175 communicates w after the
+// sweep); and T of the slab's lowest and highest planes, the ones its
+// neighbours need, into device memory.
+int evh_slab_temp(evh_ctx *h, void **lo_halo, void **hi_halo) {
+  if (!h) return EVH_ERR_STATE;
+  if (!h->cfg.conduction) return fail(h, EVH_ERR_STATE, "slab temperatures without conduction");
+  if (lo_halo) *lo_halo = h->zslab[0] ? h->thalo[0] : nullptr;
+  if (hi_halo) *hi_halo = h->zslab[1] ? h->thalo[1] : nullptr;
+  return EVH_OK;
+}
+
+int evh_slab_temp_faces(evh_ctx *h, void *lo_dst, void *hi_dst) {
+  if (!h) return EVH_ERR_STATE;
+  if (!h->cfg.conduction) return fail(h, EVH_ERR_STATE, "slab temperatures without conduction");
+  if (!h->t_valid) {  // T of the current state (after an upload, or a sweep of the staged kernels)
+    HIP_TRY(launch_temp(h));
+    h->t_valid = true;
+  }
+  const size_t bytes = (size_t)h->grid.nx * h->grid.ny * h->qelem;
+  const char *t = (const char *)h->tbuf;
+  if (lo_dst) HIP_TRY(hipMemcpyAsync(lo_dst, t, bytes, hipMemcpyDeviceToDevice, h->stream));
+  if (hi_dst)
+    HIP_TRY(hipMemcpyAsync(hi_dst, t + (h->grid.nz - 1) * bytes, bytes, hipMemcpyDeviceToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return EVH_OK;
 }

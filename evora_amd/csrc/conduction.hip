@@ -52,6 +52,9 @@ struct CGeo {
   int nx, ny, nz;
   long long n;
   int zchunk, tiles_x, tiles_y, nchunks, ntiles;
+  // a z-slab (evh_slab): T of the neighbours' planes next to the slab, below
+  // (tlo) and above (thi), [ny][nx] in the storage type; null: bound's rules
+  const void *tlo, *thi;
 };
 
 // linear index of the cell (gx, gy, gz) or of the interior cell that
@@ -69,6 +72,16 @@ __device__ __forceinline__ long long src_cell(const CGeo &G, bool per, int gx, i
   const int sy = G.ny == 1 ? 0 : ghost_src(gy, G.ny, per);
   const int sz = G.nz == 1 ? 0 : ghost_src(gz, G.nz, per);
   return (long long)sx + (long long)G.nx * ((long long)sy + (long long)G.ny * sz);
+}
+
+// T of the cell (gx, gy) of plane gz: from tb, or in a z-slab, for the plane
+// beyond a side with a neighbour, from that neighbour's plane (the one
+// communicate passes before thermalconduction, main.f90:173)
+template <typename S>
+__device__ __forceinline__ double t_at(const CGeo &G, bool per, const S *__restrict__ tb, int gx, int gy, int gz) {
+  if (gz < 0 && G.tlo) return double(((const S *)G.tlo)[src_cell(G, per, gx, gy, 0)]);
+  if (gz >= G.nz && G.thi) return double(((const S *)G.thi)[src_cell(G, per, gx, gy, 0)]);
+  return double(tb[src_cell(G, per, gx, gy, gz)]);
 }
 
 // Per-cell arithmetic of the pass.  temp: T of a cell from its conserved q
@@ -207,7 +220,7 @@ __global__ void __launch_bounds__(CT) k_conduct(KParams P, CGeo G, S *__restrict
 
   // tm, tc, tn: T of planes k-1, k, k+1 of the column; qn: q of plane k,
   // loaded one plane ahead
-  double tm = double(tb[src_cell(G, per, gx, gy, z0 - 1)]);
+  double tm = t_at(G, per, tb, gx, gy, z0 - 1);
   double tc = double(tb[src_cell(G, per, gx, gy, z0)]);
   S qn[6];
 #pragma unroll
@@ -220,7 +233,7 @@ __global__ void __launch_bounds__(CT) k_conduct(KParams P, CGeo G, S *__restrict
 #pragma unroll
       for (int m = 0; m < 6; m++) qn[m] = q[m * G.n + col + plane * (k + 1)];
     }
-    const double tn = double(tb[src_cell(G, per, gx, gy, k + 1)]);
+    const double tn = t_at(G, per, tb, gx, gy, k + 1);
     T[(cx + 1) + RX * (cy + 1)] = tc;
     if (ring) T[(rx + 1) + RX * (ry + 1)] = double(tb[src_cell(G, per, x0 + rx, y0 + ry, k)]);
     __syncthreads();
@@ -298,6 +311,8 @@ CGeo make_cgeo(const evh_ctx *h) {
   g.tiles_y = (g.ny + CY - 1) / CY;
   g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
   g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
+  g.tlo = h->zslab[0] ? h->thalo[0] : nullptr;
+  g.thi = h->zslab[1] ? h->thalo[1] : nullptr;
   return g;
 }
 

@@ -76,6 +76,7 @@ struct evh_ctx {
   // in the storage type, instead of the z rules of bound
   bool zslab[2] = {false, false};
   void *zhalo[2] = {nullptr, nullptr};
+  void *thalo[2] = {nullptr, nullptr};  // with conduction: T of the neighbours' adjacent planes, [ny][nx]
 
   // staged path arrays (allocated on first use)
   double *w = nullptr, *evo = nullptr, *s = nullptr;

@@ -85,6 +85,18 @@ module evora_hip
        type(c_ptr), value :: h, lo_dst, hi_dst
      end function evh_slab_faces
 
+     ! the same for T before thermalconduction (main.f90:170-175)
+     integer(c_int) function evh_slab_temp(h, lo_halo, hi_halo) bind(c, name='evh_slab_temp')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       type(c_ptr) :: lo_halo, hi_halo
+     end function evh_slab_temp
+
+     integer(c_int) function evh_slab_temp_faces(h, lo_dst, hi_dst) bind(c, name='evh_slab_temp_faces')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, lo_dst, hi_dst
+     end function evh_slab_temp_faces
+
      integer(c_int) function evh_destroy(h) bind(c, name='evh_destroy')
        import :: c_int, c_ptr
        type(c_ptr), value :: h

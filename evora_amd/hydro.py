@@ -40,7 +40,7 @@ EXPORTS = (
     "evh_last_error", "evh_event_record", "evh_event_elapsed", "evh_synchronize",
     "evh_set_tuning", "evh_get_tuning", "evh_set_timing", "evh_kernel_stats", "evh_conduction_stats", "evh_stage_stats",
     "evh_gravpotential", "evh_gravintegration", "evh_download_phi",
-    "evh_slab", "evh_slab_faces",
+    "evh_slab", "evh_slab_faces", "evh_slab_temp", "evh_slab_temp_faces",
     "evh_device_bytes", "evh_math_eval",
     "evh_batch_create", "evh_batch_destroy", "evh_batch_upload_q", "evh_batch_download_q",
     "evh_batch_set_time", "evh_batch_run", "evh_batch_times", "evh_batch_set_tuning",
@@ -117,6 +117,8 @@ def load_library():
             "evh_download_phi": [P, D],
             "evh_slab": [P, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
             "evh_slab_faces": [P, C.c_void_p, C.c_void_p],
+            "evh_slab_temp": [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+            "evh_slab_temp_faces": [P, C.c_void_p, C.c_void_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -285,6 +287,18 @@ class Hydro:
         (addresses or None): the planes communicate sends
         (parallel.f90:317-381)."""
         self._check(self.L.evh_slab_faces(self.h, lo_dst, hi_dst))
+
+    def slab_temp(self):
+        """With conduction on: the device halo buffers of T for the planes next
+        to the slab (addresses, None for a side without a neighbour)."""
+        a, b = C.c_void_p(), C.c_void_p()
+        self._check(self.L.evh_slab_temp(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def slab_temp_faces(self, lo_dst, hi_dst):
+        """T of the slab's lowest and highest planes into device buffers: what
+        the neighbours need before thermalconduction (main.f90:170-175)."""
+        self._check(self.L.evh_slab_temp_faces(self.h, lo_dst, hi_dst))
 
     def step(self, t: float, tend: float):
         """One pass of main.f90:109-155; returns (t, dt, go)."""

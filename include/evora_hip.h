@@ -136,7 +136,7 @@ int evh_run(evh_ctx *h, int nsteps, double *t, double tend, double *dt, int *don
  * drives the step through evh_primitive, evh_hydrostep (dt reduced to the
  * minimum over the slabs, reducestep parallel.f90:155-165), evh_compevo and
  * evh_sweep; evh_step and evh_run refuse a slab.  Fused modes on grids of
- * two planes or more, hydro only (no conduction or gravity). */
+ * two planes or more; not with gravity (its Poisson solve spans the grid). */
 int evh_slab(evh_ctx *h, int lo, int hi, void **lo_halo, void **hi_halo);
 /* The slab's two lowest (lo_dst) and two highest (hi_dst) planes of the
  * current q, [6][2][ny][nx] in the storage type, into device memory (a
@@ -144,6 +144,16 @@ int evh_slab(evh_ctx *h, int lo, int hi, void **lo_halo, void **hi_halo);
  * communicate sends (parallel.f90:334,348).  Either pointer may be NULL;
  * complete when it returns. */
 int evh_slab_faces(evh_ctx *h, void *lo_dst, void *hi_dst);
+/* With conduction on, the exchange main.f90:170-175 makes before
+ * thermalconduction: evh_slab_temp gives the device addresses of the halo
+ * buffers for T of the plane next to the slab on each side with a neighbour
+ * ([ny][nx], storage type; NULL for a side without), which the caller fills
+ * before each evh_thermalconduction while the context is idle;
+ * evh_slab_temp_faces copies T of the slab's lowest and highest planes of the
+ * current state into device memory (either pointer may be NULL; complete
+ * when it returns). */
+int evh_slab_temp(evh_ctx *h, void **lo_halo, void **hi_halo);
+int evh_slab_temp_faces(evh_ctx *h, void *lo_dst, void *hi_dst);
 
 /* snapshot.f90:169-172 sums of q1, q5, q6 (without the volume factor), FP64
  * compensated on the device. */

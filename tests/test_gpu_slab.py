@@ -30,7 +30,9 @@ FORMS = [(hydro.FP64, hydro.MODE_FUSED), (hydro.FP64, hydro.MODE_FUSED_EXACT), (
 
 
 def run_slabs(d, q0, nranks, nsteps, precision, mode, tend=1e30):
-    """nsteps of main.f90:109-155 on nranks z-slabs, each a context on this GPU."""
+    """nsteps of main.f90:109-175 on nranks z-slabs, each a context on this GPU:
+    with conduction on, thermalstep joins the dt reduction and T of the planes next
+    to each slab is exchanged before thermalconduction."""
     ranges = zslab_ranges(d.nz, nranks)
     periodic = d.boundtype == 2 and nranks > 1
     hs = []
@@ -39,11 +41,13 @@ def run_slabs(d, q0, nranks, nsteps, precision, mode, tend=1e30):
             h = hydro.Hydro(dataclasses.replace(d, nz=b - a), precision=precision, mode=mode)
             hs.append(h)
             h.upload_q(q0[a:b])
-        halos = []
+        halos, thalos = [], []
         for r, h in enumerate(hs):
             lo = r > 0 or periodic
             hi = r < nranks - 1 or periodic
             halos.append(h.slab(lo, hi))
+            if d.conduction:
+                thalos.append(h.slab_temp())
         t, go, n = 0.0, True, 0
         while go and n < nsteps:
             # communicate: the lowest planes to the neighbour below, the highest to the one above,
@@ -58,6 +62,8 @@ def run_slabs(d, q0, nranks, nsteps, precision, mode, tend=1e30):
                 h.primitive()
                 h.bound()
                 dt = h.hydrostep(dt)
+                if d.conduction:
+                    dt = h.thermalstep(dt)
             if tend - t <= dt:  # updatetime, timestep.f90:163-169
                 dt = tend - t
                 go = False
@@ -65,6 +71,15 @@ def run_slabs(d, q0, nranks, nsteps, precision, mode, tend=1e30):
             for h in hs:
                 h.compevo(dt)
                 h.sweep(dt)
+            if d.conduction:  # main.f90:170-175
+                for h in hs:
+                    h.synchronize()
+                for r, h in enumerate(hs):
+                    below, above = (r - 1) % nranks, (r + 1) % nranks
+                    h.slab_temp_faces(thalos[below][1] if thalos[r][0] else None,
+                                      thalos[above][0] if thalos[r][1] else None)
+                for h in hs:
+                    h.thermalconduction(dt)
             n += 1
         return np.concatenate([h.download_q() for h in hs]), t, n
     finally:
@@ -104,6 +119,20 @@ def test_slabs_equal_the_whole_grid_other_schemes(nranks, solver, limiter):
     qs, ts, _ = run_slabs(d, q0, nranks, 5, hydro.FP64, hydro.MODE_FUSED)
     qw, tw, _ = run_whole(d, q0, 5, hydro.FP64, hydro.MODE_FUSED)
     assert ts == tw
+    assert np.array_equal(qs, qw)
+
+
+@pytest.mark.parametrize("precision,mode", FORMS)
+@pytest.mark.parametrize("bt", [1, 2])
+def test_slabs_with_conduction_equal_the_whole_grid(bt, precision, mode):
+    """The hot blob with thermal conduction on, 3 slabs of 7, 6 and 6 planes, 4 steps: the temperature of the
+    neighbours' planes crosses before each conduction pass, and thermalstep's limit joins the dt minimum."""
+    p = ics.conduction_params(16, nz=19, boundtype=bt)
+    d = p.derive()
+    q0 = ics.conduction_blob(p)
+    qs, ts, ns = run_slabs(d, q0, 3, 4, precision, mode)
+    qw, tw, nw = run_whole(d, q0, 4, precision, mode)
+    assert ns == nw == 4 and ts == tw
     assert np.array_equal(qs, qw)
 
 
@@ -148,8 +177,12 @@ def test_slab_refusals():
         with pytest.raises(hydro.EvoraError) as ei:
             h.slab(True, True)
         assert ei.value.code == 8
-    pc = ics.conduction_params(12, boundtype=2)
-    with hydro.Hydro(pc.derive()) as h:
+    pg = ics.jeans_params(12, nz=8)
+    with hydro.Hydro(pg.derive()) as h:
         with pytest.raises(hydro.EvoraError) as ei:
             h.slab(True, True)
         assert ei.value.code == 8
+    with hydro.Hydro(d) as h:
+        with pytest.raises(hydro.EvoraError) as ei:
+            h.slab_temp()
+        assert ei.value.code == 10
