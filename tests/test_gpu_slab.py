@@ -21,8 +21,11 @@ import oracle as O
 from evora_amd import hydro, ics
 from evora_amd.params import zslab_ranges
 
-# imported before the library first touches the GPU, as in test_zz_no_alloc.py
-torch = pytest.importorskip("torch")
+# device buffers for the faces test; imported before the library first touches the GPU
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
 
 pytestmark = pytest.mark.gpu
 
@@ -148,6 +151,7 @@ def test_slabs_match_the_oracle():
     assert np.array_equal(qs[..., :5], qo[..., :5])
 
 
+@pytest.mark.skipif(torch is None, reason="device buffers come from torch")
 def test_slab_faces_are_the_boundary_planes():
     p = ics.blast_params(12, nz=9, tend=1e30)
     d = p.derive()
