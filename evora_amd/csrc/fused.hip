@@ -127,10 +127,10 @@ template <typename T, typename S, int SOLVER, int LIMITER>
 void launch_fused_t(evh_ctx *h, double dt) {
   constexpr int LO = sizeof(T) == 8 ? 2 : 3, HI = LO + 1;
   if (h->grid.nz == 1) {
-    // 2-D: FP64 at 2, FP32 at 4 waves per SIMD, without spills (one more
-    // wave per SIMD spills 64 / 32 B per lane and measured 10% / 27% slower
-    // at 8192^2)
-    launch_2d<T, S, SOLVER, LIMITER, sizeof(T) == 8 ? 2 : 4>(h, dt);
+    // 2-D: FP64 at 3 waves per SIMD (its row carry in LDS, 16 B/lane
+    // spilled: 3% faster than 2 waves at 8192^2), FP32 at 4 without spills
+    // (5 spill 32 B/lane and measured 4% slower)
+    launch_2d<T, S, SOLVER, LIMITER, sizeof(T) == 8 ? 3 : 4>(h, dt);
     return;
   }
   if (h->zslab[0] || h->zslab[1]) {

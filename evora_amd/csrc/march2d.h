@@ -35,10 +35,18 @@ struct Geo2 {
   int ychunk, strips, nchunks, ntiles;
 };
 
+// FP64 carries the top face state and bottom flux of row k-1 in lane-private
+// LDS rather than in registers: 168 VGPRs and 3 waves per SIMD (52 KB of LDS,
+// 3 blocks per CU) instead of 184 and 2.  FP32 keeps them in registers.
+template <typename T>
+constexpr bool kLdsCarry = sizeof(T) == 8;
+
 template <typename T>
 struct Smem2 {
   T W[5][NL];    // w of row k, every lane's column
   T XB[10][NL];  // x face (l, l+1): [0,5) left state, [5,10) right state; then the flux
+  T Y[kLdsCarry<T> ? 5 : 1][NL];  // lane-private: the top face state of row k-1
+  T F[kLdsCarry<T> ? 6 : 1][NL];  // lane-private: the flux of row k-1's bottom face
 };
 
 __device__ __forceinline__ long long col_off(const KParams &P, const Geo2 &G, int gx) {
@@ -81,6 +89,7 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
   using Ph = core::Phys<EXACT>;
   T(&W)[5][NL] = sm.W;
   T(&XB)[10][NL] = sm.XB;
+  constexpr bool LC = kLdsCarry<T>;
   const int l = threadIdx.x;
   const int strip = lin % G.strips, chunk = lin / G.strips;
   const int x0 = strip * NC;
@@ -101,8 +110,8 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
   T wm[5], wc[5], wp[5], qc[6], qn[6], tq[6];
   T rm, rc, rp;
   // carried from row k-1: its partial update q + dtx*(Fx), the flux of its
-  // bottom face, its top face state
-  T part[6], fyb[6], ytop[5];
+  // bottom face, its top face state (the last two in LDS with LC)
+  T part[6], fyb[LC ? 1 : 6], ytop[LC ? 1 : 5];
   constexpr bool kSpeedKey = !EXACT && sizeof(T) == 8;
   double vmin = kSpeedKey ? -1e-6 : 1e6;
   int err = 0, err_k = 0, nerr = 0, nerr_k = 0;
@@ -121,9 +130,17 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
   }
   fetch2(G, qin, cx + row_off(P, G, y0), pre);
 #pragma unroll
-  for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); part[m] = T(0); fyb[m] = T(0); }
+  for (int m = 0; m < 6; m++) { qc[m] = T(0); qn[m] = T(0); part[m] = T(0); }
 #pragma unroll
-  for (int m = 0; m < 5; m++) ytop[m] = T(0);
+  for (int m = 0; m < 6; m++) {
+    if constexpr (LC) sm.F[m][l] = T(0);
+    else fyb[m] = T(0);
+  }
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    if constexpr (LC) sm.Y[m][l] = T(0);
+    else ytop[m] = T(0);
+  }
 
   for (int k = y0 - 1; k <= y1; k++) {
     const bool out_row = k >= y0 && k < y1;
@@ -169,22 +186,36 @@ __device__ __forceinline__ void march2d(const KParams P, const Geo2 G, const S *
           yt[m] = fm::fmad(half, sy[m], wc[m]) + ev[m];
         }
         if (k >= y0) {
-          T fy[6];
-          const int e = Ph::template riemann<SOLVER, 2>(P, ytop, yb, fy);
+          T fy[6], yt0[5];
+#pragma unroll
+          for (int m = 0; m < 5; m++) {
+            if constexpr (LC) yt0[m] = sm.Y[m][l];
+            else yt0[m] = ytop[m];
+          }
+          const int e = Ph::template riemann<SOLVER, 2>(P, yt0, yb, fy);
           if (e && inside && !err) { err = e; err_k = k; }
           if (k - 1 >= y0 && inside) {
 #pragma unroll
-            for (int m = 0; m < 6; m++) o[m] = fm::fmad(dty, fyb[m] - fy[m], part[m]);
+            for (int m = 0; m < 6; m++) {
+              if constexpr (LC) o[m] = fm::fmad(dty, sm.F[m][l] - fy[m], part[m]);
+              else o[m] = fm::fmad(dty, fyb[m] - fy[m], part[m]);
+            }
             oc = (long long)gx + (long long)G.nx * (k - 1);
 #pragma unroll
             for (int m = 0; m < 6; m++) qout[m * G.n + oc] = S(o[m]);
             pend = true;
           }
 #pragma unroll
-          for (int m = 0; m < 6; m++) fyb[m] = fy[m];
+          for (int m = 0; m < 6; m++) {
+            if constexpr (LC) sm.F[m][l] = fy[m];
+            else fyb[m] = fy[m];
+          }
         }
 #pragma unroll
-        for (int m = 0; m < 5; m++) ytop[m] = yt[m];
+        for (int m = 0; m < 5; m++) {
+          if constexpr (LC) sm.Y[m][l] = yt[m];
+          else ytop[m] = yt[m];
+        }
       }
     }
     __syncthreads();
