@@ -577,6 +577,7 @@ def main():
     ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity form")
     ap.add_argument("--no-zslab", action="store_true", help="skip one GPU's z-slab of an 8-way split")
     ap.add_argument("--zslab-only", action="store_true", help="run only the z-slab form")
+    ap.add_argument("--two-d-only", action="store_true", help="run only the 2-D form")
     ap.add_argument("--gravity-only", action="store_true",
                     help="run only the self-gravity form and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
@@ -604,6 +605,9 @@ def main():
         return
     if args.gravity_only:
         print(json.dumps(run_gravity(args.n, args.steps, args.warmup, args.fp64_only)), flush=True)
+        return
+    if args.two_d_only:
+        print(json.dumps(run_2d(args.steps, args.warmup, args.fp64_only)), flush=True)
         return
     if args.zslab_only:
         pw = ics.blast_params(args.n, solver=2, cfl=args.cfl)
