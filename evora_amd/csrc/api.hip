@@ -288,7 +288,7 @@ int evh_create(const evh_config *cfg, evh_ctx **out) {
   // 64-plane chunks, the higher waves-per-SIMD budget: FP64 runs at 2
   // waves/SIMD, FP32 at 4)
   h->tile_ty = 12;
-  h->zchunk = 64;
+  h->zchunk = 0;  // automatic: 64 planes, fewer on small grids
   h->wpe_hi = 1;
 
   auto bail = [&](hipError_t e) {

@@ -77,16 +77,19 @@ __global__ void __launch_bounds__(256) k_cfl(KParams P, Geo G, const S *__restri
   if ((threadIdx.x & 63) == 0) atomic_min_pos(&st->vmin_bits, vmin);
 }
 
+// The z chunk when the tuning leaves it open (auto_chunk, tiling.h): 64
+// planes, shorter on small grids (128^3 in 64-plane chunks would be 176
+// blocks of 65 planes each; in 8-plane chunks it is 1408 blocks).
 Geo make_geo(const evh_ctx *h, int ty) {
   Geo g;
   g.nx = h->grid.nx;
   g.ny = h->grid.ny;
   g.nz = h->grid.nz;
   g.n = h->grid.n();
-  g.zchunk = h->zchunk > 0 ? h->zchunk : 128;
-  if (g.zchunk > g.nz) g.zchunk = g.nz;
   g.tiles_x = (g.nx + core::TX - 1) / core::TX;
   g.tiles_y = (g.ny + ty - 1) / ty;
+  g.zchunk = h->zchunk > 0 ? h->zchunk : auto_chunk((long long)g.tiles_x * g.tiles_y, g.nz, 64);
+  if (g.zchunk > g.nz) g.zchunk = g.nz;
   g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
   g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
   g.zlo = h->zslab[0] ? h->zhalo[0] : nullptr;
@@ -103,17 +106,17 @@ void launch_ty(evh_ctx *h, double dt) {
                      (S *)h->tbuf);
 }
 
-// the 2-D form: strips of 252 columns, chunks of 128 rows (the z chunk of the
-// tuning, when set), WPE waves per SIMD
+// the 2-D form: strips of 252 columns, chunks of up to 128 rows (the z chunk
+// of the tuning, when set), WPE waves per SIMD
 template <typename T, typename S, int SOLVER, int LIMITER, int WPE>
 void launch_2d(evh_ctx *h, double dt) {
   core2d::Geo2 g;
   g.nx = h->grid.nx;
   g.ny = h->grid.ny;
   g.n = h->grid.n();
-  g.ychunk = h->zchunk > 0 ? h->zchunk : 128;
-  if (g.ychunk > g.ny) g.ychunk = g.ny;
   g.strips = (g.nx + core2d::NC - 1) / core2d::NC;
+  g.ychunk = h->zchunk > 0 ? h->zchunk : auto_chunk(g.strips, g.ny, 128);
+  if (g.ychunk > g.ny) g.ychunk = g.ny;
   g.nchunks = (g.ny + g.ychunk - 1) / g.ychunk;
   g.ntiles = g.strips * g.nchunks;
   hipLaunchKernelGGL((K_FUSED2D<T, S, SOLVER, LIMITER, WPE>), dim3(xcd_blocks(g.ntiles)), dim3(core2d::NL), 0,

@@ -25,4 +25,16 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
 // blocks to launch for ntiles tiles (a multiple of the XCD count)
 inline int xcd_blocks(int ntiles) { return kXcds * ((ntiles + kXcds - 1) / kXcds); }
 
+// Planes (rows) per block of a march along n with `columns` column tiles per
+// plane: `start`, halved down to 4 while the grid would give fewer than 1024
+// blocks.  A block's march is serial, so on a small grid long chunks leave
+// most of the GPU idle and the pass takes the march's latency; short chunks
+// cost a few planes more of loads and primitives per chunk.  The results do
+// not depend on the chunk.
+inline int auto_chunk(long long columns, int n, int start) {
+  int c = start;
+  while (c > 4 && columns * ((n + c - 1) / c) < 1024) c /= 2;
+  return c;
+}
+
 }  // namespace evh

@@ -343,7 +343,8 @@ class Hydro:
         self._check(self.L.evh_synchronize(self.h))
 
     def set_tuning(self, tile_ty: int = 12, wpe_hi: bool = True, zchunk: int = 0):
-        """Launch shape of the fused kernel: tile height, occupancy target, z chunk."""
+        """Launch shape of the fused kernel: tile height, occupancy target, z chunk
+        (0: automatic, shorter chunks on small grids)."""
         self._check(self.L.evh_set_tuning(self.h, tile_ty, int(bool(wpe_hi)), zchunk))
 
     def get_tuning(self):
