@@ -182,9 +182,11 @@ hipError_t timer_end(evh_ctx *h, int stage) {
 
 // Sync and surface device-side errors.  Errors stay sticky until read here.
 int check_status(evh_ctx *h) {
+  // the status block is read behind the queued work (pinned host copy, one
+  // wait instead of a wait and a blocking copy)
+  HIP_TRY(hipMemcpyAsync(h->st_host, h->st, sizeof(Status), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   accumulate_timing(h);
-  HIP_TRY(hipMemcpy(h->st_host, h->st, sizeof(Status), hipMemcpyDeviceToHost));
   if (h->st_host->code) {
     const int code = h->st_host->code;
     const long long cell = (long long)h->st_host->cell;
