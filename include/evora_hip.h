@@ -174,7 +174,7 @@ int evh_synchronize(evh_ctx *h);
 /* Launch shape of the fused kernel (no counterpart in the reference; set by
  * the autotuner): tile height 8 or 12 columns, occupancy target (0: the
  * lower, 1: the higher of the two register budgets compiled per precision),
- * z planes per block (rows on 2-D grids; 0, the default: 64 planes or 128
+ * z planes per block (rows on 2-D grids; 0, the default: 64 planes or 32
  * rows, halved down to 4 while the grid gives fewer than 1024 blocks).
  * Invalid values are rejected.  A z-slab with halo buffers (evh_slab) runs
  * tile height 12 at the higher budget; its z chunk is the tuning's.  The
