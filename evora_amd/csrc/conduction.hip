@@ -308,7 +308,7 @@ CGeo make_cgeo(const evh_ctx *h) {
   g.n = h->grid.n();
   g.tiles_x = (g.nx + CX - 1) / CX;
   g.tiles_y = (g.ny + CY - 1) / CY;
-  g.zchunk = auto_chunk((long long)g.tiles_x * g.tiles_y, g.nz, EVH_COND_ZCHUNK);
+  g.zchunk = z_chunk((long long)g.tiles_x * g.tiles_y, g.nz, EVH_COND_ZCHUNK);
   if (g.zchunk > g.nz) g.zchunk = g.nz;
   g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
   g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;

@@ -77,19 +77,6 @@ __global__ void __launch_bounds__(256) k_cfl(KParams P, Geo G, const S *__restri
   if ((threadIdx.x & 63) == 0) atomic_min_pos(&st->vmin_bits, vmin);
 }
 
-// The z chunk of the 3-D march when the tuning leaves it open: 64 planes,
-// halved while the grid gives fewer than 2048 blocks, but not below nz/8
-// (a power of two from 4 to 64).  Measured fastest on MI355X for the
-// Kelvin-Helmholtz cube (FP64 and FP32-mixed): 8 planes at 96^3, 16 at 128^3
-// and 192^3, 32 at 256^3, 64 at 512^3; 32 for a 512 x 512 x 64 slab.  (128^3
-// in 64-plane chunks is 176 blocks of 65 planes, and most of the GPU idles.)
-int fused_chunk(long long tiles, int nz) {
-  int lo = 4;
-  while (lo < 64 && 2 * lo * 8 <= nz) lo *= 2;
-  int c = 64;
-  while (c > lo && tiles * ((nz + c - 1) / c) < 2048) c /= 2;
-  return c;
-}
 Geo make_geo(const evh_ctx *h, int ty) {
   Geo g;
   g.nx = h->grid.nx;
@@ -98,7 +85,7 @@ Geo make_geo(const evh_ctx *h, int ty) {
   g.n = h->grid.n();
   g.tiles_x = (g.nx + core::TX - 1) / core::TX;
   g.tiles_y = (g.ny + ty - 1) / ty;
-  g.zchunk = h->zchunk > 0 ? h->zchunk : fused_chunk((long long)g.tiles_x * g.tiles_y, g.nz);
+  g.zchunk = h->zchunk > 0 ? h->zchunk : z_chunk((long long)g.tiles_x * g.tiles_y, g.nz, 64);
   if (g.zchunk > g.nz) g.zchunk = g.nz;
   g.nchunks = (g.nz + g.zchunk - 1) / g.zchunk;
   g.ntiles = g.tiles_x * g.tiles_y * g.nchunks;
@@ -127,7 +114,7 @@ void launch_2d(evh_ctx *h, double dt) {
   g.strips = (g.nx + core2d::NC - 1) / core2d::NC;
   // 32 rows measured fastest at 8192^2 (FP64 2.10 ms against 2.14 with 64 and
   // 2.27 with 128; FP32-mixed 0.94 with 32 or 64)
-  g.ychunk = h->zchunk > 0 ? h->zchunk : auto_chunk(g.strips, g.ny, 32);
+  g.ychunk = h->zchunk > 0 ? h->zchunk : y_chunk(g.strips, g.ny, 32);
   if (g.ychunk > g.ny) g.ychunk = g.ny;
   g.nchunks = (g.ny + g.ychunk - 1) / g.ychunk;
   g.ntiles = g.strips * g.nchunks;

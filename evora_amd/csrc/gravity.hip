@@ -671,7 +671,7 @@ GGeo make_ggeo(const evh_ctx *h) {
   g.n = h->grid.n();
   g.tiles_x = (g.nx + GX - 1) / GX;
   g.tiles_y = (g.ny + GY - 1) / GY;
-  g.zchunk = auto_chunk((long long)g.tiles_x * g.tiles_y, g.nz, kZChunk);
+  g.zchunk = z_chunk((long long)g.tiles_x * g.tiles_y, g.nz, kZChunk);
   if (g.zchunk > g.nz) g.zchunk = g.nz;
   g.ntiles = g.tiles_x * g.tiles_y * ((g.nz + g.zchunk - 1) / g.zchunk);
   return g;

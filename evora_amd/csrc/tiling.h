@@ -25,15 +25,34 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
 // blocks to launch for ntiles tiles (a multiple of the XCD count)
 inline int xcd_blocks(int ntiles) { return kXcds * ((ntiles + kXcds - 1) / kXcds); }
 
-// Planes (rows) per block of a march along n with `columns` column tiles per
-// plane: `start`, halved down to 4 while the grid would give fewer than 1024
-// blocks.  A block's march is serial, so on a small grid long chunks leave
-// most of the GPU idle and the pass takes the march's latency; short chunks
-// cost a few planes more of loads and primitives per chunk.  The results do
-// not depend on the chunk.
-inline int auto_chunk(long long columns, int n, int start) {
+// Chunk lengths when the tuning leaves them open.  A block's march is
+// serial, so on a small grid long chunks leave most of the GPU idle and a
+// pass takes the march's latency; short chunks cost a few planes more of
+// loads and primitives per chunk.  The results do not depend on the chunk.
+//
+// z_chunk: planes per block of a 3-D march (the fused step, conduction, the
+// gravity force) with `columns` column tiles per plane: `start`, halved
+// while the grid would give fewer than 2048 blocks, but not below nz/8 (a
+// power of two, 4 at least).  Measured fastest on MI355X: the fused step of
+// the Kelvin-Helmholtz cube at 8 planes for 96^3, 16 for 128^3 and 192^3,
+// 32 for 256^3, 64 for 512^3 and 32 for a 512 x 512 x 64 slab; the
+// conduction pass of a 128^3 hot blob 0.082 ms at 16 planes against 0.207 ms
+// at 4 (1024 blocks of 4 planes).
+inline int z_chunk(long long columns, int nz, int start) {
+  int lo = 4;
+  while (lo < start && 2 * lo * 8 <= nz) lo *= 2;
   int c = start;
-  while (c > 4 && columns * ((n + c - 1) / c) < 1024) c /= 2;
+  while (c > lo && columns * ((nz + c - 1) / c) < 2048) c /= 2;
+  return c;
+}
+
+// y_chunk: rows per block of the 2-D march with `strips` row strips:
+// `start`, halved down to 4 while the grid would give fewer than 1024
+// blocks (a row is a block's whole strip, so short chunks cost little:
+// 128^2 measured fastest at 4 rows, 8192^2 at 32).
+inline int y_chunk(long long strips, int ny, int start) {
+  int c = start;
+  while (c > 4 && strips * ((ny + c - 1) / c) < 1024) c /= 2;
   return c;
 }
 
