@@ -25,7 +25,9 @@ CPU hot path timed on the host's cores on a bounded sample.
 The single-pass 2-D form is timed on the 8192^2 shear layer of
 para/kevinhelmholtz (forms.two_d), and one GPU's share of the blast split
 into 8 z-slabs, a 512 x 512 x 64 slab with its ghost planes exchanged
-through halo buffers every step, on its own (forms.zslab).  Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
+through halo buffers every step, on its own (forms.zslab); small grids (the
+reference's 128^2 Kelvin-Helmholtz grid, 64^3 and 128^3 blasts) give their
+wall time per step (forms.small_grids).  Unless --no-accuracy is given, each FP32-mixed configuration (the blast with
 HLLC and with HLL, Kelvin-Helmholtz 256^3 at CFL 0.5 for 50 steps, the
 cosmological-like field 256^3 for 100 steps, the Jeans box with gravity and
 the hot blob with conduction at 256^3 for 50 steps, and the batched
@@ -448,6 +450,37 @@ def run_2d(steps, warmup, fp64_only, n=8192):
     return out
 
 
+def run_small(steps=200, warmup=5):
+    """Wall time per step of the step loop on small grids, where a step is
+    bounded by the march's latency and the host's per-step work rather than by
+    throughput: the reference's own 128^2 Kelvin-Helmholtz grid
+    (para/kevinhelmholtz) and 64^3 and 128^3 blasts, FP64 fused."""
+    from evora_amd import hydro, ics
+    from evora_amd.params import HydroParams
+
+    cases = [("kh2d_128", HydroParams(nx=128, ny=128, nz=1, solver=2, limiter=2, boundtype=2, cfl=0.5,
+                                      atomic=False)),
+             ("blast_64", ics.blast_params(64, solver=2, cfl=0.4)), ("blast_128", ics.blast_params(128, solver=2, cfl=0.4))]
+    out = {}
+    for name, p in cases:
+        d = p.derive()
+        q0 = ics.kh_reference(p) if name.startswith("kh") else ics.blast(p)
+        with hydro.Hydro(d) as h:
+            h.upload_q(q0)
+            t = 0.0
+            for _ in range(warmup):
+                t, dt, go = h.step(t, 1e30)
+            h.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                t, dt, go = h.step(t, 1e30)
+            h.synchronize()
+            ms = 1e3 * (time.perf_counter() - t0) / steps
+        out[name] = dict(ms_per_step=ms, value=d.ncells / (ms * 1e-3), cells=d.ncells, steps=steps)
+        log(f"small grid {name}: {ms:.4f} ms/step")
+    return out
+
+
 def cpu_baseline(n, cores, nsteps=2):
     """The reference's own hot path (oracle/_ref/evref, compiled from the
     reference Fortran at -O3) on `cores` processes, each advancing one z-slab
@@ -578,6 +611,7 @@ def main():
     ap.add_argument("--no-zslab", action="store_true", help="skip one GPU's z-slab of an 8-way split")
     ap.add_argument("--zslab-only", action="store_true", help="run only the z-slab form")
     ap.add_argument("--two-d-only", action="store_true", help="run only the 2-D form")
+    ap.add_argument("--no-small", action="store_true", help="skip the small-grid step times")
     ap.add_argument("--gravity-only", action="store_true",
                     help="run only the self-gravity form and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
@@ -666,6 +700,8 @@ def main():
         forms["gravity"] = run_gravity(args.n, args.steps, args.warmup, args.fp64_only)
     if not args.no_zslab:
         forms["zslab"] = run_zslab(args.n, 8, args.steps, args.warmup, fp64["kernel_ms"])
+    if not args.no_small:
+        forms["small_grids"] = run_small()
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_blast{args.n}_fp64.json")
     if os.path.exists(tpath):
