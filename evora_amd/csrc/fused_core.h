@@ -506,7 +506,6 @@ __device__ __forceinline__ void march(const KParams P, const Geo G, const S *__r
 
     // ---- S2: slopes, predictor and face states of plane k
     T sx[5], sy[5], sz[5], ev[5];
-    T fz[6];
     if (owns_col) {
 #pragma unroll
     for (int m = 0; m < 5; m++) {
