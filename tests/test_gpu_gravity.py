@@ -338,3 +338,25 @@ def test_gravity_dropin_2d(exact):
     O.Oracle(p.derive()).run(qo, 6)
     for c in range(6):
         assert rel_l1(qh[..., c], qo[..., c]) <= 1e-10, c
+
+
+@pytest.mark.parametrize("shape,mode", [((24, 20, 1), hydro.MODE_FUSED), ((24, 20, 1), hydro.MODE_FUSED_EXACT),
+                                        ((48, 1, 1), hydro.MODE_STAGED)])
+def test_gravity_with_conduction_reduced_dimensions(shape, mode):
+    """main.f90:157-175 on a 2-D (the single-pass 2-D sweep) and a 1-D grid (the staged kernels): sweep, gravity,
+    then conduction, 3 steps against the oracle."""
+    nx, ny, nz = shape
+    p = ics.conduction_params(nx, ny=ny, nz=nz, boundtype=2, gravity=True)
+    d = dataclasses.replace(p.derive(), grav=5e4)
+    q0 = ics.conduction_blob(p)
+    qo = q0.copy()
+    O.Oracle(d).run(qo, 3)
+    with hydro.Hydro(d, mode=mode) as h:
+        h.upload_q(q0)
+        h.run(3)
+        q = h.download_q()
+    qh = q0.copy()
+    O.Oracle(dataclasses.replace(d, gravity=0)).run(qh, 3)
+    assert rel_l1(qo[..., 1], qh[..., 1]) > 1e-6  # gravity changed the run
+    for c in range(6):
+        assert rel_l1(q[..., c], qo[..., c]) <= 1e-10, c

@@ -139,6 +139,19 @@ def test_slabs_with_conduction_equal_the_whole_grid(bt, precision, mode):
     assert np.array_equal(qs, qw)
 
 
+def test_slabs_fp32_mixed_dual_energy():
+    """The FP32-mixed form with dual energy (its FP64 entropy branch on the neighbours' planes too), 4 slabs,
+    periodic: the whole-grid run bit for bit."""
+    p = ics.blast_params(20, nz=16, boundtype=2, solver=2, limiter=1, cfl=0.4, tend=1e30)
+    p = dataclasses.replace(p, dualenergy=True, de_thresh=0.1)
+    d = p.derive()
+    q0 = ics.blast(p)
+    qs, ts, _ = run_slabs(d, q0, 4, 5, hydro.FP32_MIXED, hydro.MODE_FUSED)
+    qw, tw, _ = run_whole(d, q0, 5, hydro.FP32_MIXED, hydro.MODE_FUSED)
+    assert ts == tw
+    assert np.array_equal(qs, qw)
+
+
 def test_slabs_match_the_oracle():
     """Two slabs against the oracle's whole-grid run: the exact form bit for bit in q1-q5 and t."""
     p = ics.blast_params(16, nz=24, boundtype=1, solver=2, cfl=0.4, tend=1e30)
